@@ -1,0 +1,319 @@
+/*
+ * kinfu.h — C ABI of the MI355X (gfx950) KinectFusion hot path.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns a
+ * kf_status (never exits the process) and is asynchronous on the stream it is
+ * given (NULL = the device's default stream) unless its comment says "blocking".
+ * Handles are thread-compatible (one handle per host thread), not thread-safe.
+ *
+ * Units: metres, seconds, pixels.  Camera frame: x right, y down, z forward
+ * (standard pinhole).  Poses are row-major 4x4, T_wc maps camera -> world.
+ *
+ * Each group names the reference interface it replaces (paths relative to the
+ * OpenFusion source tree).
+ */
+#ifndef KINFU_H_
+#define KINFU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes.  Replace cutilSafeCall's print-and-exit
+ * (Source/include/Utils/cutil_inline_runtime.h:256-263). */
+typedef enum kf_status {
+  KF_OK = 0,
+  KF_ERR_INVALID_ARGUMENT = 1,
+  KF_ERR_HIP = 2,
+  KF_ERR_OUT_OF_MEMORY = 3,
+  KF_ERR_NOT_READY = 4,
+  KF_ERR_IO = 5,
+  KF_ERR_UNSUPPORTED = 6
+} kf_status;
+
+typedef void* kf_stream; /* a hipStream_t; NULL = default stream */
+
+/* Last error text of the calling thread (never NULL). */
+const char* kf_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Core types.  Replace CameraParams / Intrinsic (Source/include/BasicTypes.h:74-103)
+ * and the compile-time constants of Source/include/Constants.h:10-55. */
+typedef struct kf_intrinsics {
+  float fx, fy, cx, cy; /* pixels; pixel (x,y) centre is at integer coordinates */
+  int32_t width, height;
+} kf_intrinsics;
+
+/* Intrinsics of pyramid level `level` (2^level downsampling, pixel-centre aligned). */
+kf_intrinsics kf_intrinsics_level(const kf_intrinsics* k0, int32_t level);
+
+typedef enum kf_voxel_format {
+  KF_VOXEL_F32 = 0, /* 8 B/voxel: {float tsdf/T, float weight}            */
+  KF_VOXEL_F16 = 1  /* 4 B/voxel: {half tsdf/T (low 16 bits), u16 weight} */
+} kf_voxel_format;
+
+/* Volume: n^3 voxels, z fastest (index = (x*n + y)*n + z), voxel (i,j,k) centre at
+ * origin + (i+0.5, j+0.5, k+0.5) * extent/n.  Replaces the tsdf/weight pair
+ * allocated in RunApp::initCudaPointers (Source/src/MainApp.cpp:596-597). */
+typedef struct kf_volume_desc {
+  int32_t n;
+  float extent_m;
+  float origin_m[3];
+  float trunc_m;
+  float max_weight;
+  int32_t format; /* kf_voxel_format */
+} kf_volume_desc;
+
+typedef struct kf_volume kf_volume;
+
+/* Device pose.  Replaces d_currentTransform / d_invCurrentTransform /
+ * d_cameraPosition (Source/src/MainApp.cpp:613-633).  Lives in device memory;
+ * T_wc is authoritative (FP64), wc/cw are its FP32 roundings used by kernels. */
+typedef struct kf_posef {
+  float r[9]; /* row-major rotation */
+  float t[3];
+} kf_posef;
+
+typedef struct kf_pose {
+  double T_wc[16]; /* camera -> world, row-major, FP64 */
+  kf_posef wc;     /* (float) of T_wc's upper 3x4                          */
+  kf_posef cw;     /* (float) of the FP64 rigid inverse (R^T, -R^T t)        */
+} kf_pose;
+
+/* One pyramid level of a depth / vertex / normal map set (planar SoA, row-major,
+ * width*height floats each).  Invalid vertex/normal = NaN in every component;
+ * invalid depth = 0.  Replaces CameraData / PointInfo (Source/include/BasicTypes.h:43-52). */
+typedef struct kf_level_maps {
+  int32_t width, height;
+  float* depth; /* metres (live maps only; NULL in model maps) */
+  float *vx, *vy, *vz;
+  float *nx, *ny, *nz;
+} kf_level_maps;
+
+#define KF_MAX_LEVELS 3
+
+typedef struct kf_frame_maps {
+  int32_t levels;
+  float* raylen; /* level 0 only: distance camera->measured surface along the pixel ray, 0 = invalid */
+  kf_level_maps level[KF_MAX_LEVELS];
+} kf_frame_maps;
+
+/* ------------------------------------------------------------------------ */
+/* Device memory and streams (for callers without their own HIP runtime binding). */
+kf_status kf_device_count(int32_t* count);
+kf_status kf_device_malloc(void** d_ptr, size_t bytes);
+kf_status kf_device_free(void* d_ptr);
+kf_status kf_host_malloc_pinned(void** h_ptr, size_t bytes);
+kf_status kf_host_free_pinned(void* h_ptr);
+kf_status kf_memcpy_htod(void* d_dst, const void* h_src, size_t bytes, kf_stream s);
+kf_status kf_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes, kf_stream s);
+kf_status kf_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes, kf_stream s);
+kf_status kf_memset_d8(void* d_dst, int value, size_t bytes, kf_stream s);
+kf_status kf_stream_create(kf_stream* out);
+kf_status kf_stream_destroy(kf_stream s);
+kf_status kf_stream_synchronize(kf_stream s); /* blocking */
+kf_status kf_device_synchronize(void);        /* blocking */
+
+/* Timing events (hipEvent) recorded on the stream the kernels run on. */
+typedef void* kf_event;
+kf_status kf_event_create(kf_event* out);
+kf_status kf_event_destroy(kf_event e);
+kf_status kf_event_record(kf_event e, kf_stream s);
+kf_status kf_event_elapsed_ms(kf_event start, kf_event stop, float* ms); /* blocking */
+
+/* ------------------------------------------------------------------------ */
+/* Maps: caller-owned structs of device pointers (allocated here for convenience). */
+kf_status kf_maps_alloc(const kf_intrinsics* k0, int32_t levels, int32_t with_depth, kf_frame_maps* out);
+kf_status kf_maps_free(kf_frame_maps* maps);
+
+/* ------------------------------------------------------------------------ */
+/* Poses.  Replaces General::UpdateTransforms' host Eigen inverse + d_updateTransforms
+ * (Source/src/General.cpp:14-40, Source/src/General.cu:57-62). */
+kf_status kf_pose_upload(kf_pose* d_pose, const double T_wc[16], kf_stream s);
+kf_status kf_pose_download(double T_wc[16], const kf_pose* d_pose, kf_stream s); /* blocking */
+/* Host helper: fills the FP32 fields of a host kf_pose from its T_wc. */
+void kf_pose_finish_host(kf_pose* h_pose);
+
+/* ------------------------------------------------------------------------ */
+/* Volume.  kf_volume_reset replaces General::InitVoxelArray / d_initVoxelArray
+ * (Source/src/General.cpp:42-49, Source/src/General.cu:27-50): tsdf = +1 (i.e. +T), w = 0. */
+kf_status kf_volume_create(const kf_volume_desc* desc, kf_volume** out);
+kf_status kf_volume_destroy(kf_volume* vol);
+kf_status kf_volume_reset(kf_volume* vol, kf_stream s);
+kf_status kf_volume_get_desc(const kf_volume* vol, kf_volume_desc* desc);
+kf_status kf_volume_data(const kf_volume* vol, void** d_voxels, size_t* bytes);
+/* Occupancy statistics over every voxel (reads the whole volume once):
+ * d_out (device, 4 doubles) = {observed voxels (w > 0), sum of w, sum of f over
+ * observed voxels, voxels with f < 0 and w > 0}. */
+kf_status kf_volume_stats(const kf_volume* vol, double* d_out, kf_stream s);
+
+/* ------------------------------------------------------------------------ */
+/* Depth preprocessing: uint16 mm depth -> raylen (L0), bilateral-filtered depth,
+ * 3-level pyramid, camera-frame vertex and normal maps per level.  Replaces
+ * CameraHandler::updateCameraData and its two kernels
+ * (Source/src/CameraSupport/CameraHandler.cu:65-143). */
+typedef struct kf_preprocess_params {
+  float depth_scale;  /* metres per input unit (0.001 for mm) */
+  float min_depth_m;
+  float max_depth_m;
+  int32_t bilateral_radius; /* window half-size in pixels (<= 4) */
+  float sigma_space_px;
+  float sigma_range_m;
+  float pyramid_gate_m; /* block-average gate: |d - centre| < gate */
+} kf_preprocess_params;
+
+kf_status kf_preprocess(const kf_intrinsics* k0, const kf_preprocess_params* p,
+                        const uint16_t* d_depth, kf_frame_maps* live, kf_stream s);
+
+/* ------------------------------------------------------------------------ */
+/* TSDF integration sweep.  Replaces VolumeIntegrator::Integrate / d_integrateVolume
+ * (Source/src/VolumeIntegrator.cu:86-201).  A voxel is updated iff its centre
+ * projects (round-half-up) into [2,W-2)x[2,H-2), its camera z is in
+ * [min_depth, max_depth], the pixel's raylen is valid and sdf >= -T.
+ * d_updated (nullable) receives += number of updated voxels (uint64);
+ * d_update_mask (nullable, n^3/32 uint32 words, caller-zeroed) gets one bit per
+ * updated voxel (bit (idx & 31) of word idx >> 5). */
+typedef struct kf_integrate_params {
+  float min_depth_m;
+  float max_depth_m;
+} kf_integrate_params;
+
+kf_status kf_integrate(kf_volume* vol, const kf_intrinsics* k0, const kf_integrate_params* p,
+                       const float* d_raylen, const kf_pose* d_pose,
+                       unsigned long long* d_updated, uint32_t* d_update_mask, kf_stream s);
+
+/* ------------------------------------------------------------------------ */
+/* Raycast.  Replaces Raycaster::Raycast / d_raycast (Source/src/Raycaster.cu:97-242)
+ * with trilinear sampling, linear zero-crossing refinement and trilinear
+ * central-difference gradient normals.  Writes world-frame vertex/normal maps of
+ * level 0 and subsamples (x2) into the coarser levels of `model`.
+ * d_steps (nullable) receives += total march steps over all rays (uint64). */
+typedef struct kf_raycast_params {
+  float min_depth_m;  /* rays start at this camera-z depth */
+  float step_far;     /* step (in units of T) while the sample is +1 (free space) */
+  float step_near;    /* step (in voxels) inside the truncation band */
+  int32_t max_steps;
+} kf_raycast_params;
+
+kf_status kf_raycast(const kf_volume* vol, const kf_intrinsics* k0, const kf_raycast_params* p,
+                     const kf_pose* d_pose, kf_frame_maps* model,
+                     unsigned long long* d_steps, kf_stream s);
+
+/* ------------------------------------------------------------------------ */
+/* ICP tracker.  Replaces ICP::Iterate_ICP, ICP::FindCorresponding and
+ * ICP::BuildNewTransformMatrix (Source/src/ICP.cpp:84-182, Source/src/ICP.cu:116-340).
+ * Projective association model->live, point-to-plane rows A=[m x d, m],
+ * b = m.(s-d); per-block partial sums of the 21+6 normal-equation terms plus
+ * sum b^2, inlier and candidate counts; the last block of each iteration sums
+ * the partials in FP64, solves by FP64 Cholesky and composes the Euler-ZYX
+ * increment T <- dT * T in FP64 on the device (no host round trip). */
+typedef struct kf_icp_params {
+  int32_t levels;                     /* 1..3 */
+  int32_t iterations[KF_MAX_LEVELS];  /* per level, [0] = finest; run coarse -> fine */
+  float dist_thresh_m;
+  float normal_thresh_deg;
+  int32_t compensated;                /* 1: Kahan FP32 partials (default); 0: plain FP32 partials */
+} kf_icp_params;
+
+/* Per-call tracking statistics (device struct; read back with kf_memcpy_dtoh). */
+typedef struct kf_icp_stats {
+  int32_t status;        /* 0 ok; 1 a system was singular (pose of that iteration kept, later iterations skipped) */
+  int32_t iterations;    /* iterations that produced an update */
+  int32_t inliers;       /* associations in the last executed iteration */
+  int32_t candidates;    /* pixels with valid model vertex/normal and live depth, last iteration */
+  double residual_rms;   /* sqrt(sum b^2 / inliers), last executed iteration (m) */
+  double level_residual[KF_MAX_LEVELS];
+  int32_t level_inliers[KF_MAX_LEVELS];
+  int32_t skip;          /* input flag: when nonzero the call does nothing (device-side gating) */
+  int32_t pad_;
+} kf_icp_stats;
+
+typedef struct kf_tracker kf_tracker;
+
+kf_status kf_tracker_create(const kf_intrinsics* k0, const kf_icp_params* p, kf_tracker** out);
+kf_status kf_tracker_destroy(kf_tracker* t);
+/* Tracks `live` against `model` starting at *d_guess; writes the result to *d_out
+ * (may alias d_guess).  d_stats (device, nullable): written with the call's stats;
+ * if d_stats->skip is nonzero at execution time, the call leaves *d_out untouched. */
+kf_status kf_icp_track(kf_tracker* t, const kf_frame_maps* live, const kf_frame_maps* model,
+                       const kf_pose* d_guess, kf_pose* d_out, kf_icp_stats* d_stats, kf_stream s);
+
+/* One association + reduction pass at `level` with pose *d_est (no solve);
+ * writes the 30 FP64 sums {AtA upper 21 (row-major i<=j), Atb 6, sum b^2,
+ * inliers, candidates} to d_sums (device, 30 doubles) and, if d_inlier_mask is
+ * not NULL, one byte per model pixel (1 = associated).  For testing and tools. */
+kf_status kf_icp_reduce(kf_tracker* t, int32_t level, const kf_frame_maps* live,
+                        const kf_frame_maps* model, const kf_pose* d_est,
+                        double* d_sums, uint8_t* d_inlier_mask, kf_stream s);
+
+/* Constant-velocity guess on the device: T_guess = T1 * (T0^-1 * T1), FP64.
+ * (prev2 = frame k-2, prev1 = frame k-1.) */
+kf_status kf_pose_constant_velocity(const kf_pose* d_prev2, const kf_pose* d_prev1,
+                                    kf_pose* d_guess, kf_stream s);
+
+/* ------------------------------------------------------------------------ */
+/* Frame pipeline: the per-frame driver of RunApp::display
+ * (Source/src/MainApp.cpp:774-970) in KinectFusion order
+ * preprocess -> ICP (guess: constant velocity or learned prior) -> integrate -> raycast,
+ * recorded once into a hipGraph and replayed per frame. */
+typedef struct kf_pipeline_desc {
+  kf_intrinsics intr;
+  kf_volume_desc volume;
+  kf_preprocess_params pre;
+  kf_integrate_params integ;
+  kf_raycast_params ray;
+  kf_icp_params icp;          /* plain ICP (from the constant-velocity guess) */
+  kf_icp_params icp_prior;    /* warm-started ICP used when a prior is attached */
+  float fallback_residual_ratio; /* prior fallback: residual > ratio * running median */
+  float fallback_min_inlier_frac; /* prior fallback: inliers / candidates below this */
+  int32_t use_graph;          /* 1: record and replay a hipGraph per frame */
+} kf_pipeline_desc;
+
+typedef struct kf_frame_stats {
+  int32_t frame;
+  int32_t used_prior;   /* the learned prior supplied the guess */
+  int32_t fell_back;    /* prior result rejected; plain ICP result used */
+  int32_t pad_;
+  kf_icp_stats icp;          /* ICP that produced the pose */
+  kf_icp_stats icp_prior;    /* warm-started ICP (valid if used_prior) */
+  unsigned long long updated_voxels; /* stage-timing mode only: voxels updated by the sweep */
+  unsigned long long raycast_steps;  /* stage-timing mode only: total march steps */
+} kf_frame_stats;
+
+typedef struct kf_pipeline kf_pipeline;
+
+kf_status kf_pipeline_default_desc(kf_pipeline_desc* d);
+/* s = NULL: the pipeline creates (and owns) a non-blocking stream of its own. */
+kf_status kf_pipeline_create(const kf_pipeline_desc* d, kf_stream s, kf_pipeline** out);
+kf_status kf_pipeline_destroy(kf_pipeline* p);
+/* Clears the volume, sets frame 0's pose (host T_wc) and the frame counter. */
+kf_status kf_pipeline_reset(kf_pipeline* p, const double T0_wc[16]);
+/* Processes one uint16 depth frame already in device memory. */
+kf_status kf_pipeline_process(kf_pipeline* p, const uint16_t* d_depth);
+kf_status kf_pipeline_get_pose(kf_pipeline* p, double T_wc[16]);          /* blocking */
+/* Poses of every frame since the last reset (device log, up to KF_TRAJ_CAPACITY
+ * frames, written by the pipeline itself): copies min(frames, max_frames) T_wc
+ * into T_out (host, 16 doubles each) and the logged frame count into *frames. */
+#define KF_TRAJ_CAPACITY 8192
+kf_status kf_pipeline_get_trajectory(kf_pipeline* p, double* T_out, int32_t max_frames,
+                                     int32_t* frames); /* blocking */
+kf_status kf_pipeline_get_stats(kf_pipeline* p, kf_frame_stats* stats);   /* blocking */
+kf_status kf_pipeline_get_volume(kf_pipeline* p, kf_volume** vol);
+kf_status kf_pipeline_get_maps(kf_pipeline* p, kf_frame_maps* live, kf_frame_maps* model);
+kf_status kf_pipeline_get_stream(kf_pipeline* p, kf_stream* s);
+/* Stage-timing mode (eager launches, hipEvents on the pipeline stream, voxel and
+ * march-step counters).  Times (ms) of the last processed frame:
+ * [preprocess, guess+icp, column spans, sweep kernel, raycast]. */
+#define KF_STAGES 5
+kf_status kf_pipeline_set_stage_timing(kf_pipeline* p, int32_t on);
+kf_status kf_pipeline_get_stage_ms(kf_pipeline* p, float ms[KF_STAGES]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KINFU_H_ */

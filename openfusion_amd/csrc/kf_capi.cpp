@@ -1,0 +1,391 @@
+// kf_capi.cpp — the extern "C" surface of libkinfu: argument checking, handle
+// ownership, memory/stream helpers and the per-frame pipeline driver.
+//
+// The pipeline replaces RunApp::display's tracking branch (Source/src/MainApp.cpp:774-970)
+// in KinectFusion order preprocess -> track -> integrate -> raycast, entirely on
+// one HIP stream with no host synchronisation; after the first tracked frame the
+// sequence is recorded into a hipGraph and replayed per frame.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+
+#include "kf_internal.hpp"
+#include "kf_pose_math.hpp"
+
+namespace kf {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+kf_status hip_fail(hipError_t e, const char* what) {
+  set_error(std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")");
+  return e == hipErrorOutOfMemory ? KF_ERR_OUT_OF_MEMORY : KF_ERR_HIP;
+}
+
+kf_status check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, what);
+  return KF_OK;
+}
+
+void pose_finish(kf_pose* p) { pose_finish_fields(p); }
+
+static kf_status check_intrinsics(const kf_intrinsics* k) {
+  KF_CHECK_ARG(k != nullptr, "intrinsics is NULL");
+  KF_CHECK_ARG(k->width >= 16 && k->height >= 16 && k->width <= 8192 && k->height <= 8192,
+               "image size must be 16..8192");
+  KF_CHECK_ARG(k->fx > 0.0f && k->fy > 0.0f, "focal lengths must be positive");
+  return KF_OK;
+}
+
+static kf_status check_volume_desc(const kf_volume_desc* d) {
+  KF_CHECK_ARG(d != nullptr, "volume desc is NULL");
+  KF_CHECK_ARG(d->n >= 8 && d->n <= 2048 && d->n % 8 == 0, "n must be a multiple of 8 in 8..2048");
+  KF_CHECK_ARG(d->extent_m > 0.0f, "extent must be positive");
+  KF_CHECK_ARG(d->trunc_m > 0.0f, "truncation must be positive");
+  KF_CHECK_ARG(d->max_weight >= 1.0f && d->max_weight <= 65535.0f, "max_weight must be 1..65535");
+  KF_CHECK_ARG(d->format == KF_VOXEL_F32 || d->format == KF_VOXEL_F16, "unknown voxel format");
+  return KF_OK;
+}
+
+static kf_status check_icp_params(const kf_icp_params* p) {
+  KF_CHECK_ARG(p != nullptr, "icp params is NULL");
+  KF_CHECK_ARG(p->levels >= 1 && p->levels <= KF_MAX_LEVELS, "icp levels must be 1..3");
+  for (int l = 0; l < p->levels; ++l)
+    KF_CHECK_ARG(p->iterations[l] >= 0 && p->iterations[l] <= 100, "iterations must be 0..100");
+  KF_CHECK_ARG(p->dist_thresh_m > 0.0f, "dist_thresh must be positive");
+  KF_CHECK_ARG(p->normal_thresh_deg > 0.0f && p->normal_thresh_deg < 90.0f, "normal_thresh must be in (0,90)");
+  return KF_OK;
+}
+
+}  // namespace kf
+
+using namespace kf;
+
+// --------------------------------------------------------------------------
+extern "C" const char* kf_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" kf_intrinsics kf_intrinsics_level(const kf_intrinsics* k0, int32_t level) {
+  kf_intrinsics k = *k0;
+  const float scale = 1.0f / (float)(1 << level);
+  k.fx = k0->fx * scale;
+  k.fy = k0->fy * scale;
+  k.cx = (k0->cx + 0.5f) * scale - 0.5f;
+  k.cy = (k0->cy + 0.5f) * scale - 0.5f;
+  k.width = k0->width >> level;
+  k.height = k0->height >> level;
+  return k;
+}
+
+// --------------------------------------------------------------------------
+extern "C" kf_status kf_device_count(int32_t* count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    *count = 0;
+    return hip_fail(e, "hipGetDeviceCount");
+  }
+  *count = n;
+  return KF_OK;
+}
+extern "C" kf_status kf_device_malloc(void** p, size_t bytes) {
+  KF_CHECK_ARG(p != nullptr, "out pointer is NULL");
+  KF_HIP(hipMalloc(p, bytes));
+  return KF_OK;
+}
+extern "C" kf_status kf_device_free(void* p) {
+  if (p) KF_HIP(hipFree(p));
+  return KF_OK;
+}
+extern "C" kf_status kf_host_malloc_pinned(void** p, size_t bytes) {
+  KF_CHECK_ARG(p != nullptr, "out pointer is NULL");
+  KF_HIP(hipHostMalloc(p, bytes, hipHostMallocDefault));
+  return KF_OK;
+}
+extern "C" kf_status kf_host_free_pinned(void* p) {
+  if (p) KF_HIP(hipHostFree(p));
+  return KF_OK;
+}
+extern "C" kf_status kf_memcpy_htod(void* d, const void* h, size_t n, kf_stream s) {
+  KF_HIP(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, S(s)));
+  return KF_OK;
+}
+extern "C" kf_status kf_memcpy_dtoh(void* h, const void* d, size_t n, kf_stream s) {
+  KF_HIP(hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, S(s)));
+  KF_HIP(hipStreamSynchronize(S(s)));
+  return KF_OK;
+}
+extern "C" kf_status kf_memcpy_dtod(void* d, const void* src, size_t n, kf_stream s) {
+  KF_HIP(hipMemcpyAsync(d, src, n, hipMemcpyDeviceToDevice, S(s)));
+  return KF_OK;
+}
+extern "C" kf_status kf_memset_d8(void* d, int v, size_t n, kf_stream s) {
+  KF_HIP(hipMemsetAsync(d, v, n, S(s)));
+  return KF_OK;
+}
+extern "C" kf_status kf_stream_create(kf_stream* out) {
+  hipStream_t st;
+  KF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  *out = st;
+  return KF_OK;
+}
+extern "C" kf_status kf_stream_destroy(kf_stream s) {
+  if (s) KF_HIP(hipStreamDestroy(S(s)));
+  return KF_OK;
+}
+extern "C" kf_status kf_stream_synchronize(kf_stream s) {
+  KF_HIP(hipStreamSynchronize(S(s)));
+  return KF_OK;
+}
+extern "C" kf_status kf_device_synchronize(void) {
+  KF_HIP(hipDeviceSynchronize());
+  return KF_OK;
+}
+extern "C" kf_status kf_event_create(kf_event* out) {
+  hipEvent_t e;
+  KF_HIP(hipEventCreate(&e));
+  *out = e;
+  return KF_OK;
+}
+extern "C" kf_status kf_event_destroy(kf_event e) {
+  if (e) KF_HIP(hipEventDestroy(reinterpret_cast<hipEvent_t>(e)));
+  return KF_OK;
+}
+extern "C" kf_status kf_event_record(kf_event e, kf_stream s) {
+  KF_HIP(hipEventRecord(reinterpret_cast<hipEvent_t>(e), S(s)));
+  return KF_OK;
+}
+extern "C" kf_status kf_event_elapsed_ms(kf_event a, kf_event b, float* ms) {
+  KF_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(b)));
+  KF_HIP(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(a), reinterpret_cast<hipEvent_t>(b)));
+  return KF_OK;
+}
+
+// --------------------------------------------------------------------------
+extern "C" kf_status kf_maps_alloc(const kf_intrinsics* k0, int32_t levels, int32_t with_depth,
+                                   kf_frame_maps* out) {
+  KF_TRY(check_intrinsics(k0));
+  KF_CHECK_ARG(out != nullptr, "out is NULL");
+  KF_CHECK_ARG(levels >= 1 && levels <= KF_MAX_LEVELS, "levels must be 1..3");
+  memset(out, 0, sizeof(*out));
+  out->levels = levels;
+  size_t total = 0;
+  for (int l = 0; l < levels; ++l) {
+    kf_intrinsics kl = kf_intrinsics_level(k0, l);
+    total += (size_t)kl.width * kl.height * (with_depth ? 7 : 6);
+  }
+  if (with_depth) total += (size_t)k0->width * k0->height;
+  float* base = nullptr;
+  KF_HIP(hipMalloc(&base, total * sizeof(float)));
+  float* p = base;
+  if (with_depth) {
+    out->raylen = p;
+    p += (size_t)k0->width * k0->height;
+  }
+  for (int l = 0; l < levels; ++l) {
+    kf_intrinsics kl = kf_intrinsics_level(k0, l);
+    const size_t P = (size_t)kl.width * kl.height;
+    kf_level_maps& m = out->level[l];
+    m.width = kl.width;
+    m.height = kl.height;
+    if (with_depth) {
+      m.depth = p;
+      p += P;
+    }
+    m.vx = p; p += P;
+    m.vy = p; p += P;
+    m.vz = p; p += P;
+    m.nx = p; p += P;
+    m.ny = p; p += P;
+    m.nz = p; p += P;
+  }
+  // Start with every pixel invalid (NaN vertices/normals, zero depth).
+  KF_HIP(hipMemset(base, 0xff, total * sizeof(float)));
+  if (with_depth) {
+    KF_HIP(hipMemset(out->raylen, 0, (size_t)k0->width * k0->height * sizeof(float)));
+    for (int l = 0; l < levels; ++l)
+      KF_HIP(hipMemset(out->level[l].depth, 0,
+                       (size_t)out->level[l].width * out->level[l].height * sizeof(float)));
+  }
+  return KF_OK;
+}
+
+extern "C" kf_status kf_maps_free(kf_frame_maps* m) {
+  KF_CHECK_ARG(m != nullptr, "maps is NULL");
+  void* base = m->raylen ? (void*)m->raylen : (void*)m->level[0].vx;
+  if (m->raylen == nullptr && m->level[0].depth) base = m->level[0].depth;
+  if (base) KF_HIP(hipFree(base));
+  memset(m, 0, sizeof(*m));
+  return KF_OK;
+}
+
+// --------------------------------------------------------------------------
+extern "C" void kf_pose_finish_host(kf_pose* p) { pose_finish_fields(p); }
+
+extern "C" kf_status kf_pose_upload(kf_pose* d_pose, const double T[16], kf_stream s) {
+  KF_CHECK_ARG(d_pose && T, "pose is NULL");
+  kf_pose h;
+  memcpy(h.T_wc, T, sizeof(h.T_wc));
+  pose_finish_fields(&h);
+  KF_HIP(hipMemcpyAsync(d_pose, &h, sizeof(h), hipMemcpyHostToDevice, S(s)));
+  KF_HIP(hipStreamSynchronize(S(s)));  // h lives on this stack frame
+  return KF_OK;
+}
+
+extern "C" kf_status kf_pose_download(double T[16], const kf_pose* d_pose, kf_stream s) {
+  KF_CHECK_ARG(d_pose && T, "pose is NULL");
+  kf_pose h;
+  KF_HIP(hipMemcpyAsync(&h, d_pose, sizeof(h), hipMemcpyDeviceToHost, S(s)));
+  KF_HIP(hipStreamSynchronize(S(s)));
+  memcpy(T, h.T_wc, sizeof(h.T_wc));
+  return KF_OK;
+}
+
+extern "C" kf_status kf_pose_constant_velocity(const kf_pose* d_prev2, const kf_pose* d_prev1,
+                                               kf_pose* d_guess, kf_stream s) {
+  return launch_pose_constant_velocity(d_prev2, d_prev1, d_guess, S(s));
+}
+
+// --------------------------------------------------------------------------
+extern "C" kf_status kf_volume_create(const kf_volume_desc* d, kf_volume** out) {
+  KF_TRY(check_volume_desc(d));
+  KF_CHECK_ARG(out != nullptr, "out is NULL");
+  kf_volume* v = new kf_volume();
+  v->desc = *d;
+  const size_t n = (size_t)d->n;
+  v->bytes = n * n * n * (d->format == KF_VOXEL_F16 ? 4 : 8);
+  hipError_t e = hipMalloc(&v->d_voxels, v->bytes);
+  if (e != hipSuccess) {
+    delete v;
+    return hip_fail(e, "hipMalloc(volume)");
+  }
+  e = hipMalloc(&v->d_spans, n * n * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    (void)hipFree(v->d_voxels);
+    delete v;
+    return hip_fail(e, "hipMalloc(spans)");
+  }
+  kf_status st = launch_volume_reset(v, nullptr);
+  if (st == KF_OK) {
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) st = hip_fail(e, "volume init");
+  }
+  if (st != KF_OK) {
+    (void)hipFree(v->d_voxels);
+    (void)hipFree(v->d_spans);
+    delete v;
+    return st;
+  }
+  *out = v;
+  return KF_OK;
+}
+
+extern "C" kf_status kf_volume_destroy(kf_volume* v) {
+  if (!v) return KF_OK;
+  (void)hipFree(v->d_voxels);
+  (void)hipFree(v->d_spans);
+  delete v;
+  return KF_OK;
+}
+
+extern "C" kf_status kf_volume_reset(kf_volume* v, kf_stream s) {
+  KF_CHECK_ARG(v != nullptr, "volume is NULL");
+  return launch_volume_reset(v, S(s));
+}
+
+extern "C" kf_status kf_volume_get_desc(const kf_volume* v, kf_volume_desc* d) {
+  KF_CHECK_ARG(v && d, "NULL argument");
+  *d = v->desc;
+  return KF_OK;
+}
+
+extern "C" kf_status kf_volume_data(const kf_volume* v, void** p, size_t* bytes) {
+  KF_CHECK_ARG(v && p && bytes, "NULL argument");
+  *p = v->d_voxels;
+  *bytes = v->bytes;
+  return KF_OK;
+}
+
+extern "C" kf_status kf_volume_stats(const kf_volume* v, double* d_out, kf_stream s) {
+  KF_CHECK_ARG(v != nullptr, "volume is NULL");
+  return launch_volume_stats(v, d_out, S(s));
+}
+
+// --------------------------------------------------------------------------
+extern "C" kf_status kf_preprocess(const kf_intrinsics* k0, const kf_preprocess_params* p,
+                                   const uint16_t* d_depth, kf_frame_maps* live, kf_stream s) {
+  KF_TRY(check_intrinsics(k0));
+  KF_CHECK_ARG(p && live, "NULL argument");
+  return launch_preprocess(*k0, *p, d_depth, *live, S(s));
+}
+
+extern "C" kf_status kf_integrate(kf_volume* v, const kf_intrinsics* k0, const kf_integrate_params* p,
+                                  const float* d_raylen, const kf_pose* d_pose,
+                                  unsigned long long* d_updated, uint32_t* d_mask, kf_stream s) {
+  KF_CHECK_ARG(v && p, "NULL argument");
+  KF_TRY(check_intrinsics(k0));
+  return launch_integrate(v, *k0, *p, d_raylen, d_pose, d_updated, d_mask, S(s));
+}
+
+extern "C" kf_status kf_raycast(const kf_volume* v, const kf_intrinsics* k0, const kf_raycast_params* p,
+                                const kf_pose* d_pose, kf_frame_maps* model,
+                                unsigned long long* d_steps, kf_stream s) {
+  KF_CHECK_ARG(v && p && model, "NULL argument");
+  KF_TRY(check_intrinsics(k0));
+  return launch_raycast(v, *k0, *p, d_pose, *model, d_steps, S(s));
+}
+
+// --------------------------------------------------------------------------
+extern "C" kf_status kf_tracker_create(const kf_intrinsics* k0, const kf_icp_params* p, kf_tracker** out) {
+  KF_TRY(check_intrinsics(k0));
+  KF_TRY(check_icp_params(p));
+  KF_CHECK_ARG(out != nullptr, "out is NULL");
+  kf_tracker* t = new kf_tracker();
+  t->k0 = *k0;
+  t->params = *p;
+  t->max_blocks = 1024;
+  hipError_t e = hipMalloc(&t->d_partials, (size_t)t->max_blocks * 64 * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&t->d_ctrl, sizeof(kf_icp_ctrl));
+  if (e == hipSuccess) e = hipMalloc(&t->d_work, sizeof(kf_pose));
+  if (e == hipSuccess) e = hipMalloc(&t->d_stats_tmp, sizeof(kf_icp_stats));
+  if (e == hipSuccess) e = hipMemset(t->d_ctrl, 0, sizeof(kf_icp_ctrl));
+  if (e == hipSuccess) e = hipMemset(t->d_stats_tmp, 0, sizeof(kf_icp_stats));
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipFree(t->d_partials);
+    (void)hipFree(t->d_ctrl);
+    (void)hipFree(t->d_work);
+    (void)hipFree(t->d_stats_tmp);
+    delete t;
+    return hip_fail(e, "tracker allocation");
+  }
+  *out = t;
+  return KF_OK;
+}
+
+extern "C" kf_status kf_tracker_destroy(kf_tracker* t) {
+  if (!t) return KF_OK;
+  (void)hipFree(t->d_partials);
+  (void)hipFree(t->d_ctrl);
+  (void)hipFree(t->d_work);
+  (void)hipFree(t->d_stats_tmp);
+  delete t;
+  return KF_OK;
+}
+
+extern "C" kf_status kf_icp_track(kf_tracker* t, const kf_frame_maps* live, const kf_frame_maps* model,
+                                  const kf_pose* d_guess, kf_pose* d_out, kf_icp_stats* d_stats,
+                                  kf_stream s) {
+  KF_CHECK_ARG(t && live && model, "NULL argument");
+  return launch_icp_track(t, *live, *model, d_guess, d_out, d_stats, S(s));
+}
+
+extern "C" kf_status kf_icp_reduce(kf_tracker* t, int32_t level, const kf_frame_maps* live,
+                                   const kf_frame_maps* model, const kf_pose* d_est, double* d_sums,
+                                   uint8_t* d_mask, kf_stream s) {
+  KF_CHECK_ARG(t && live && model, "NULL argument");
+  return launch_icp_reduce(t, level, *live, *model, d_est, d_sums, d_mask, S(s));
+}

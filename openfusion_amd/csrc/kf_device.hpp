@@ -1,0 +1,91 @@
+// kf_device.hpp — device-side helpers shared by the gfx950 kernels.
+//
+// All arithmetic here is written as explicit IEEE float operations in a fixed
+// order; the library is compiled with -ffp-contract=off and HIP's default
+// correctly rounded fp32 division and square root, so a host restatement that
+// evaluates the same expressions in the same order produces the same bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kinfu.h"
+
+namespace kf {
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+struct f3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 sub(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// R * v (+ t): ((r0*x + r1*y) + r2*z) + t0, row by row.
+__device__ __forceinline__ f3 rot(const float* r, f3 v) {
+  return f3{(r[0] * v.x + r[1] * v.y) + r[2] * v.z, (r[3] * v.x + r[4] * v.y) + r[5] * v.z,
+            (r[6] * v.x + r[7] * v.y) + r[8] * v.z};
+}
+__device__ __forceinline__ f3 xform(const kf_posef& p, f3 v) {
+  f3 a = rot(p.r, v);
+  return f3{a.x + p.t[0], a.y + p.t[1], a.z + p.t[2]};
+}
+
+__device__ __forceinline__ bool finite3(f3 v) { return v.x == v.x && v.y == v.y && v.z == v.z; }
+
+// Round half up, as x - floor(x) < 0.5 ? floor(x) : ceil(x) (result kept as float).
+__device__ __forceinline__ float round_half_up(float a) {
+  float fl = floorf(a);
+  return (a - fl) < 0.5f ? fl : ceilf(a);
+}
+
+// e^-x for x >= 0 from a base-2 range reduction and a degree-8 Taylor
+// polynomial of e^-y on y in [0, ln 2); returns 0 for x >= 80 or NaN.
+__device__ __forceinline__ float exp_neg(float x) {
+  if (!(x < 80.0f)) return 0.0f;
+  // n = floor(x / ln2); r = x - n ln2 by Cody-Waite (ln2_hi has 16 significant
+  // bits, so n * ln2_hi and x - n * ln2_hi are exact for n < 128).
+  const float n = floorf(x * 1.44269504f);
+  const float y = (x - n * 0.693145752f) - n * 1.42860677e-06f;
+  float p = 2.48015873e-05f;      // 1/8!
+  p = p * y - 1.98412698e-04f;    // 1/7!
+  p = p * y + 1.38888889e-03f;    // 1/6!
+  p = p * y - 8.33333333e-03f;    // 1/5!
+  p = p * y + 4.16666667e-02f;    // 1/4!
+  p = p * y - 1.66666667e-01f;    // 1/3!
+  p = p * y + 5.00000000e-01f;    // 1/2!
+  p = p * y - 1.0f;
+  p = p * y + 1.0f;
+  return ldexpf(p, -(int)n);
+}
+
+__device__ __forceinline__ float nanf_() { return __builtin_nanf(""); }
+
+// Packed FP16 voxel: low 16 bits = half(tsdf/T), high 16 bits = uint16 weight.
+__device__ __forceinline__ float half_bits_to_float(uint32_t h) {
+  _Float16 v = __builtin_bit_cast(_Float16, (uint16_t)h);
+  return (float)v;
+}
+__device__ __forceinline__ uint32_t float_to_half_bits(float f) {
+  _Float16 v = (_Float16)f;  // v_cvt_f16_f32, round to nearest even
+  return (uint32_t)__builtin_bit_cast(uint16_t, v);
+}
+
+// Wave-level sum of a float with the DPP/permute ladder, result valid in all lanes.
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+}  // namespace kf

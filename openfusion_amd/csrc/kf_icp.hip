@@ -1,0 +1,404 @@
+// kf_icp.hip — projective point-to-plane ICP on gfx950, one launch per iteration.
+//
+// Replaces d_find_corresponding / ICP::FindCorresponding (Source/src/ICP.cu:116-262),
+// the host block sum, determinant guard and Eigen LLT of ICP::Iterate_ICP
+// (Source/src/ICP.cpp:84-182) and the host pose update of
+// ICP::BuildNewTransformMatrix (Source/src/ICP.cu:266-340).
+//
+// Per model pixel (same association direction and gate structure as the
+// reference): the model vertex s / normal m (world) is moved into the live camera
+// with the current estimate, projected (round half up), and the live vertex /
+// normal there is brought to world (d, n).  The pixel is an inlier iff the live
+// depth at the model pixel is valid (ICP.cu:145-149), |d-s| < dist_thresh and
+// cos_thresh < n.m < 1+1e-6.  Row: A = [m x d, m], b = m.(s-d); after the solve
+// of (A^T A) x = A^T b the rotation is -x[0..2] (ICP.cpp:162-168).
+//
+// Reduction: every thread accumulates the 28 FP32 terms (21 A^TA, 6 A^Tb, b^2)
+// over a grid-stride run of pixels; waves combine with xor-shuffle butterflies
+// and blocks through LDS.  In compensated mode each term is carried as a
+// (sum, error) pair updated with the error-free TwoSum, so a block partial is
+// accurate to ~2^-48 relative; plain mode uses ordinary FP32 sums.  Blocks write
+// their partials, and the last block to arrive (device-scope counter) adds them
+// in FP64 in block order, solves by FP64 Cholesky, composes the Euler-ZYX
+// increment onto the FP64 pose and refreshes the FP32 pose copies, so the next
+// iteration's launch reads the new pose without any host round trip.
+#include "kf_device.hpp"
+#include "kf_internal.hpp"
+#include "kf_pose_math.hpp"
+
+namespace kf {
+namespace {
+
+constexpr int kTerms = 28;   // 21 + 6 + 1 float terms
+constexpr int kSlot = 64;    // floats per block partial: 28 (hi,lo) + 2 int counts
+constexpr int kThreads = 256;
+constexpr double kPivotRelEps = 1e-9;
+constexpr int kMinInliers = 6;
+
+struct IcpArgs {
+  const float *lvx, *lvy, *lvz, *lnx, *lny, *lnz;  // live level maps (camera frame)
+  const float *mvx, *mvy, *mvz, *mnx, *mny, *mnz;  // model level maps (world frame)
+  int W, H;
+  float fx, fy, cx, cy;
+  float dist_thr, cos_thr;
+  kf_pose* pose;
+  float* partials;
+  kf_icp_ctrl* ctrl;
+  kf_icp_stats* stats;
+  double* sums_out;
+  uint8_t* mask;
+  int level;
+  int mode;  // 0: track (solve + compose), 1: reduce only
+};
+
+__device__ __forceinline__ void two_sum(float& s, float& c, float v) {
+  const float t = s + v;
+  const float bp = t - s;
+  const float e = (s - (t - bp)) + (v - bp);
+  s = t;
+  c = c + e;
+}
+
+// Evaluates the association of model pixel i.  Returns 0 = not a candidate,
+// 1 = candidate but rejected, 2 = inlier (A, b filled).
+__device__ __forceinline__ int associate(const IcpArgs& a, const kf_posef& wc, const kf_posef& cw,
+                                         int i, float* A, float& b) {
+  const float lz_here = a.lvz[i];
+  const f3 s = f3{a.mvx[i], a.mvy[i], a.mvz[i]};
+  const f3 m = f3{a.mnx[i], a.mny[i], a.mnz[i]};
+  if (!(lz_here == lz_here) || !finite3(s) || !finite3(m)) return 0;
+  const f3 sc = xform(cw, s);
+  if (!(sc.z > 1e-6f)) return 1;
+  const float iz = 1.0f / sc.z;
+  const float u = (a.fx * sc.x + a.cx * sc.z) * iz;
+  const float v = (a.fy * sc.y + a.cy * sc.z) * iz;
+  const float ru = round_half_up(u), rv = round_half_up(v);
+  if (!(ru >= 0.0f && ru < (float)a.W && rv >= 0.0f && rv < (float)a.H)) return 1;
+  const int j = (int)rv * a.W + (int)ru;
+  const f3 dc = f3{a.lvx[j], a.lvy[j], a.lvz[j]};
+  const f3 nc = f3{a.lnx[j], a.lny[j], a.lnz[j]};
+  if (!finite3(dc) || !finite3(nc)) return 1;
+  const f3 d = xform(wc, dc);
+  const f3 n = rot(wc.r, nc);
+  const f3 df = sub(d, s);
+  if (!(sqrtf(dot(df, df)) < a.dist_thr)) return 1;
+  const float cs = dot(n, m);
+  if (!(cs > a.cos_thr && cs < 1.0f + 1e-6f)) return 1;
+  A[0] = m.y * d.z - m.z * d.y;
+  A[1] = m.z * d.x - m.x * d.z;
+  A[2] = m.x * d.y - m.y * d.x;
+  A[3] = m.x;
+  A[4] = m.y;
+  A[5] = m.z;
+  b = (m.x * (s.x - d.x) + m.y * (s.y - d.y)) + m.z * (s.z - d.z);
+  return 2;
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(kThreads) void icp_iteration(IcpArgs a) {
+  __shared__ float red_s[4][kTerms];
+  __shared__ float red_c[4][kTerms];
+  __shared__ int red_n[4][2];
+  __shared__ double fin[8][32];
+  __shared__ int is_last;
+  if (a.mode == 0 && a.ctrl->stop) return;  // uniform over the grid
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const kf_posef wc = a.pose->wc, cw = a.pose->cw;
+  float s[kTerms], c[kTerms];
+#pragma unroll
+  for (int k = 0; k < kTerms; ++k) {
+    s[k] = 0.0f;
+    c[k] = 0.0f;
+  }
+  int n_in = 0, n_cand = 0;
+  const int P = a.W * a.H;
+  for (int i = blockIdx.x * kThreads + tid; i < P; i += gridDim.x * kThreads) {
+    float A[6], b;
+    const int r = associate(a, wc, cw, i, A, b);
+    if (a.mask) a.mask[i] = (uint8_t)(r == 2);
+    if (r == 0) continue;
+    ++n_cand;
+    if (r != 2) continue;
+    ++n_in;
+    int k = 0;
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+#pragma unroll
+      for (int q = p; q < 6; ++q) {
+        const float v = A[p] * A[q];
+        if (COMP) two_sum(s[k], c[k], v); else s[k] = s[k] + v;
+        ++k;
+      }
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+      const float v = A[p] * b;
+      if (COMP) two_sum(s[21 + p], c[21 + p], v); else s[21 + p] = s[21 + p] + v;
+    }
+    if (COMP) two_sum(s[27], c[27], b * b); else s[27] = s[27] + b * b;
+  }
+
+  // Wave butterfly: every lane ends with the same (sum, error) pair.
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < kTerms; ++k) {
+      const float so = __shfl_xor(s[k], off, kWave);
+      if (COMP) {
+        const float co = __shfl_xor(c[k], off, kWave);
+        float t = s[k], e = 0.0f;
+        two_sum(t, e, so);
+        s[k] = t;
+        c[k] = (c[k] + co) + e;
+      } else {
+        s[k] = s[k] + so;
+      }
+    }
+  }
+  n_in = wave_sum_i(n_in);
+  n_cand = wave_sum_i(n_cand);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < kTerms; ++k) {
+      red_s[wave][k] = s[k];
+      red_c[wave][k] = c[k];
+    }
+    red_n[wave][0] = n_in;
+    red_n[wave][1] = n_cand;
+  }
+  __syncthreads();
+  float* slot = a.partials + (size_t)blockIdx.x * kSlot;
+  if (tid < kTerms) {
+    float S = red_s[0][tid], C = red_c[0][tid];
+    for (int w = 1; w < 4; ++w) {
+      if (COMP) {
+        float e = 0.0f;
+        two_sum(S, e, red_s[w][tid]);
+        C = (C + red_c[w][tid]) + e;
+      } else {
+        S = S + red_s[w][tid];
+      }
+    }
+    slot[2 * tid] = S;
+    slot[2 * tid + 1] = COMP ? C : 0.0f;
+  } else if (tid == kTerms) {
+    const int ni = (red_n[0][0] + red_n[1][0]) + (red_n[2][0] + red_n[3][0]);
+    const int nc = (red_n[0][1] + red_n[1][1]) + (red_n[2][1] + red_n[3][1]);
+    slot[56] = __int_as_float(ni);
+    slot[57] = __int_as_float(nc);
+  }
+
+  // Last-block-done: publish the partial, count arrivals at device scope.
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) {
+    const unsigned prev = __hip_atomic_fetch_add(&a.ctrl->arrive, 1u, __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+    is_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+
+  // FP64 cross-block sum: thread (g, k) adds blocks g, g+8, ... of term k in order.
+  const int k = tid & 31, g = tid >> 5;
+  double acc = 0.0;
+  if (k < kTerms) {
+    for (int bb = g; bb < (int)gridDim.x; bb += 8) {
+      const float* sl = a.partials + (size_t)bb * kSlot;
+      acc = (acc + (double)sl[2 * k]) + (double)sl[2 * k + 1];
+    }
+  } else if (k < kTerms + 2) {
+    for (int bb = g; bb < (int)gridDim.x; bb += 8)
+      acc = acc + (double)__float_as_int(a.partials[(size_t)bb * kSlot + 56 + (k - kTerms)]);
+  }
+  fin[g][k] = acc;
+  __syncthreads();
+  if (tid < 32) {
+    double t = fin[0][tid];
+    for (int gg = 1; gg < 8; ++gg) t = t + fin[gg][tid];
+    fin[0][tid] = t;
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  const double* sum = fin[0];
+  const double inl = sum[28], cand = sum[29];
+  if (a.mode == 1) {
+    for (int q = 0; q < 30; ++q) a.sums_out[q] = sum[q];
+  } else {
+    kf_icp_stats* st = a.stats;
+    const int ni = (int)inl;
+    st->inliers = ni;
+    st->candidates = (int)cand;
+    st->residual_rms = ni > 0 ? sqrt(sum[27] / inl) : 0.0;
+    st->level_residual[a.level] = st->residual_rms;
+    st->level_inliers[a.level] = ni;
+    double x[6];
+    const bool ok = ni >= kMinInliers && cholesky_solve6(sum, sum + 21, kPivotRelEps, x);
+    if (ok) {
+      double D[16], Tn[16];
+      euler_zyx(-x[0], -x[1], -x[2], x[3], x[4], x[5], D);
+      mat4_mul(D, a.pose->T_wc, Tn);
+      kf_pose np;
+      for (int q = 0; q < 16; ++q) np.T_wc[q] = Tn[q];
+      np.T_wc[12] = 0.0;
+      np.T_wc[13] = 0.0;
+      np.T_wc[14] = 0.0;
+      np.T_wc[15] = 1.0;
+      pose_finish_fields(&np);
+      *a.pose = np;
+      a.ctrl->executed = a.ctrl->executed + 1;
+    } else {
+      a.ctrl->stop = 1;
+      st->status = 1;
+    }
+  }
+  __hip_atomic_store(&a.ctrl->arrive, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void icp_begin(const kf_pose* guess, kf_pose* work, kf_icp_ctrl* ctrl,
+                          const kf_icp_stats* user, kf_icp_stats* st) {
+  if (threadIdx.x != 0) return;
+  const int skip = user ? user->skip : 0;
+  *work = *guess;
+  ctrl->stop = skip ? 2 : 0;
+  ctrl->executed = 0;
+  kf_icp_stats z = {};
+  *st = z;
+}
+
+__global__ void icp_end(const kf_pose* work, kf_pose* out, const kf_icp_ctrl* ctrl,
+                        kf_icp_stats* st, kf_icp_stats* user) {
+  if (threadIdx.x != 0) return;
+  if (ctrl->stop == 2) return;  // skipped call: outputs untouched
+  *out = *work;
+  st->iterations = ctrl->executed;
+  if (user && user != st) {
+    kf_icp_stats c = *st;
+    c.skip = 0;
+    *user = c;
+  }
+}
+
+__global__ void pose_const_velocity(const kf_pose* p2, const kf_pose* p1, kf_pose* g) {
+  if (threadIdx.x != 0) return;
+  double Ti[16], Dl[16], G[16];
+  rigid_inverse(p2->T_wc, Ti);
+  mat4_mul(Ti, p1->T_wc, Dl);
+  mat4_mul(p1->T_wc, Dl, G);
+  kf_pose np;
+  for (int q = 0; q < 16; ++q) np.T_wc[q] = G[q];
+  pose_finish_fields(&np);
+  *g = np;
+}
+
+__global__ void pose_copy(const kf_pose* src, kf_pose* dst) {
+  if (threadIdx.x == 0) *dst = *src;
+}
+
+IcpArgs level_args(const kf_tracker* t, int l, const kf_frame_maps& live, const kf_frame_maps& model) {
+  IcpArgs a = {};
+  const kf_level_maps& L = live.level[l];
+  const kf_level_maps& M = model.level[l];
+  kf_intrinsics kl = kf_intrinsics_level(&t->k0, l);
+  a.lvx = L.vx; a.lvy = L.vy; a.lvz = L.vz; a.lnx = L.nx; a.lny = L.ny; a.lnz = L.nz;
+  a.mvx = M.vx; a.mvy = M.vy; a.mvz = M.vz; a.mnx = M.nx; a.mny = M.ny; a.mnz = M.nz;
+  a.W = kl.width;
+  a.H = kl.height;
+  a.fx = kl.fx;
+  a.fy = kl.fy;
+  a.cx = kl.cx;
+  a.cy = kl.cy;
+  a.dist_thr = t->params.dist_thresh_m;
+  a.cos_thr = (float)cos((double)t->params.normal_thresh_deg * 3.14159265358979323846 / 180.0);
+  a.partials = t->d_partials;
+  a.ctrl = t->d_ctrl;
+  a.level = l;
+  return a;
+}
+
+int level_blocks(const kf_tracker* t, int W, int H, int l) {
+  const int ppt = (l == 0) ? 2 : 1;
+  int b = div_up(W * H, kThreads * ppt);
+  return b < t->max_blocks ? b : t->max_blocks;
+}
+
+kf_status check_maps(const kf_tracker* t, const kf_frame_maps& live, const kf_frame_maps& model,
+                     int levels) {
+  KF_CHECK_ARG(live.levels >= levels && model.levels >= levels, "maps have fewer levels than ICP");
+  for (int l = 0; l < levels; ++l) {
+    kf_intrinsics kl = kf_intrinsics_level(&t->k0, l);
+    const kf_level_maps& L = live.level[l];
+    const kf_level_maps& M = model.level[l];
+    KF_CHECK_ARG(L.width == kl.width && L.height == kl.height, "live level size mismatch");
+    KF_CHECK_ARG(M.width == kl.width && M.height == kl.height, "model level size mismatch");
+    KF_CHECK_ARG(L.vx && L.vy && L.vz && L.nx && L.ny && L.nz, "live plane is NULL");
+    KF_CHECK_ARG(M.vx && M.vy && M.vz && M.nx && M.ny && M.nz, "model plane is NULL");
+  }
+  return KF_OK;
+}
+
+}  // namespace
+
+kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_frame_maps& model,
+                           const kf_pose* d_guess, kf_pose* d_out, kf_icp_stats* d_stats,
+                           hipStream_t s) {
+  KF_CHECK_ARG(d_guess && d_out, "guess/out pose is NULL");
+  const kf_icp_params& p = t->params;
+  KF_TRY(check_maps(t, live, model, p.levels));
+  kf_icp_stats* st = t->d_stats_tmp;
+  hipLaunchKernelGGL(icp_begin, dim3(1), dim3(64), 0, s, d_guess, t->d_work, t->d_ctrl,
+                     (const kf_icp_stats*)d_stats, st);
+  KF_TRY(check_launch("icp_begin"));
+  for (int l = p.levels - 1; l >= 0; --l) {
+    IcpArgs a = level_args(t, l, live, model);
+    a.pose = t->d_work;
+    a.stats = st;
+    a.mode = 0;
+    const int blocks = level_blocks(t, a.W, a.H, l);
+    for (int it = 0; it < p.iterations[l]; ++it) {
+      if (p.compensated)
+        hipLaunchKernelGGL(icp_iteration<true>, dim3(blocks), dim3(kThreads), 0, s, a);
+      else
+        hipLaunchKernelGGL(icp_iteration<false>, dim3(blocks), dim3(kThreads), 0, s, a);
+      KF_TRY(check_launch("icp_iteration"));
+    }
+  }
+  hipLaunchKernelGGL(icp_end, dim3(1), dim3(64), 0, s, (const kf_pose*)t->d_work, d_out,
+                     (const kf_icp_ctrl*)t->d_ctrl, st, d_stats);
+  return check_launch("icp_end");
+}
+
+kf_status launch_icp_reduce(kf_tracker* t, int level, const kf_frame_maps& live,
+                            const kf_frame_maps& model, const kf_pose* d_est, double* d_sums,
+                            uint8_t* d_mask, hipStream_t s) {
+  KF_CHECK_ARG(level >= 0 && level < KF_MAX_LEVELS, "level out of range");
+  KF_CHECK_ARG(d_est && d_sums, "pose/sums is NULL");
+  KF_TRY(check_maps(t, live, model, level + 1));
+  IcpArgs a = level_args(t, level, live, model);
+  a.pose = const_cast<kf_pose*>(d_est);  // read only in reduce mode
+  a.stats = t->d_stats_tmp;
+  a.sums_out = d_sums;
+  a.mask = d_mask;
+  a.mode = 1;
+  const int blocks = level_blocks(t, a.W, a.H, level);
+  if (t->params.compensated)
+    hipLaunchKernelGGL(icp_iteration<true>, dim3(blocks), dim3(kThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL(icp_iteration<false>, dim3(blocks), dim3(kThreads), 0, s, a);
+  return check_launch("icp_iteration(reduce)");
+}
+
+kf_status launch_pose_constant_velocity(const kf_pose* d_prev2, const kf_pose* d_prev1,
+                                        kf_pose* d_guess, hipStream_t s) {
+  KF_CHECK_ARG(d_prev2 && d_prev1 && d_guess, "pose is NULL");
+  hipLaunchKernelGGL(pose_const_velocity, dim3(1), dim3(64), 0, s, d_prev2, d_prev1, d_guess);
+  return check_launch("pose_const_velocity");
+}
+
+kf_status launch_pose_copy(const kf_pose* d_src, kf_pose* d_dst, hipStream_t s) {
+  hipLaunchKernelGGL(pose_copy, dim3(1), dim3(64), 0, s, d_src, d_dst);
+  return check_launch("pose_copy");
+}
+
+}  // namespace kf

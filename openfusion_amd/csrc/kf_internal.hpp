@@ -1,0 +1,98 @@
+// kf_internal.hpp — host-side internals of libkinfu (handle layouts, error plumbing,
+// kernel launchers shared between translation units).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "kinfu.h"
+
+namespace kf {
+
+void set_error(const std::string& msg);
+kf_status hip_fail(hipError_t e, const char* what);
+
+#define KF_HIP(call)                                        \
+  do {                                                      \
+    hipError_t e_ = (call);                                 \
+    if (e_ != hipSuccess) return ::kf::hip_fail(e_, #call); \
+  } while (0)
+
+#define KF_CHECK_ARG(cond, msg)                   \
+  do {                                            \
+    if (!(cond)) {                                \
+      ::kf::set_error(std::string("invalid argument: ") + (msg)); \
+      return KF_ERR_INVALID_ARGUMENT;             \
+    }                                             \
+  } while (0)
+
+#define KF_TRY(call)                 \
+  do {                               \
+    kf_status s_ = (call);           \
+    if (s_ != KF_OK) return s_;      \
+  } while (0)
+
+inline hipStream_t S(kf_stream s) { return reinterpret_cast<hipStream_t>(s); }
+inline int div_up(int a, int b) { return (a + b - 1) / b; }
+
+// Launches the last queued kernel's error check (launch-configuration errors only).
+kf_status check_launch(const char* what);
+
+}  // namespace kf
+
+struct kf_volume {
+  kf_volume_desc desc;
+  void* d_voxels = nullptr;  // n^3 * voxel_bytes
+  size_t bytes = 0;
+  uint32_t* d_spans = nullptr;  // per (x,y) column: z0 | z1 << 16, conservative frustum span
+};
+
+// ICP scratch and per-iteration control block (device memory).
+struct kf_icp_ctrl {
+  unsigned int arrive;  // blocks finished in the current iteration (reset by the last block)
+  int stop;             // set when a system was singular: later iterations skip
+  int executed;         // iterations that produced an update
+  int pad;
+};
+
+struct kf_tracker {
+  kf_intrinsics k0;
+  kf_icp_params params;
+  int max_blocks = 0;     // partial slots
+  float* d_partials = nullptr;  // max_blocks * 2 * 32 floats (hi, lo)
+  kf_icp_ctrl* d_ctrl = nullptr;
+  kf_pose* d_work = nullptr;     // pose being refined
+  kf_icp_stats* d_stats_tmp = nullptr;  // used when the caller passes none
+  double* d_sums_tmp = nullptr;
+};
+
+namespace kf {
+
+// Launchers implemented in the .hip translation units.
+kf_status launch_preprocess(const kf_intrinsics& k0, const kf_preprocess_params& p,
+                            const uint16_t* d_depth, kf_frame_maps& live, hipStream_t s);
+kf_status launch_volume_reset(kf_volume* v, hipStream_t s);
+kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integrate_params& p,
+                           const float* d_raylen, const kf_pose* d_pose,
+                           unsigned long long* d_updated, uint32_t* d_mask, hipStream_t s,
+                           hipEvent_t between = nullptr);
+kf_status launch_volume_stats(const kf_volume* v, double* d_out, hipStream_t s);
+kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_raycast_params& p,
+                         const kf_pose* d_pose, kf_frame_maps& model, unsigned long long* d_steps,
+                         hipStream_t s);
+kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_frame_maps& model,
+                           const kf_pose* d_guess, kf_pose* d_out, kf_icp_stats* d_stats,
+                           hipStream_t s);
+kf_status launch_icp_reduce(kf_tracker* t, int level, const kf_frame_maps& live,
+                            const kf_frame_maps& model, const kf_pose* d_est, double* d_sums,
+                            uint8_t* d_mask, hipStream_t s);
+kf_status launch_pose_constant_velocity(const kf_pose* d_prev2, const kf_pose* d_prev1,
+                                        kf_pose* d_guess, hipStream_t s);
+kf_status launch_pose_copy(const kf_pose* d_src, kf_pose* d_dst, hipStream_t s);
+
+// Host-side FP64 pose helpers (identical operation order to the device versions).
+void pose_finish(kf_pose* p);  // fills wc / cw from T_wc
+
+}  // namespace kf

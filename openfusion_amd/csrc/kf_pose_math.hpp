@@ -1,0 +1,122 @@
+// kf_pose_math.hpp — FP64 rigid-transform helpers compiled for host and device.
+//
+// Replaces the host Eigen FP32 4x4 inverse of General::UpdateTransforms and
+// ICP::Iterate_ICP (Source/src/General.cpp:27-36, Source/src/ICP.cpp:106-115) with
+// the rigid inverse (R^T, -R^T t), and Cpu_buildNewTransformMatrix
+// (Source/src/ICP.cu:266-321) with an FP64 Euler-ZYX increment left-multiplied
+// onto the pose.  Operation order is fixed and contraction is off, so host and
+// device produce identical bits.
+#pragma once
+
+#include <math.h>
+
+#include "kinfu.h"
+
+#if defined(__HIPCC__)
+#define KF_HD __host__ __device__ inline
+#else
+#define KF_HD inline
+#endif
+
+namespace kf {
+
+// Ti = inverse of the rigid transform T (row-major 4x4).
+KF_HD void rigid_inverse(const double* T, double* Ti) {
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) Ti[r * 4 + c] = T[c * 4 + r];
+    Ti[r * 4 + 3] = -((T[0 * 4 + r] * T[3] + T[1 * 4 + r] * T[7]) + T[2 * 4 + r] * T[11]);
+  }
+  Ti[12] = 0.0;
+  Ti[13] = 0.0;
+  Ti[14] = 0.0;
+  Ti[15] = 1.0;
+}
+
+// C = A * B (row-major 4x4); C must not alias A or B.
+KF_HD void mat4_mul(const double* A, const double* B, double* C) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      C[i * 4 + j] = ((A[i * 4 + 0] * B[0 * 4 + j] + A[i * 4 + 1] * B[1 * 4 + j]) +
+                      A[i * 4 + 2] * B[2 * 4 + j]) +
+                     A[i * 4 + 3] * B[3 * 4 + j];
+}
+
+// Fills the FP32 fields of p from p->T_wc.
+KF_HD void pose_finish_fields(kf_pose* p) {
+  double Ti[16];
+  rigid_inverse(p->T_wc, Ti);
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) {
+      p->wc.r[r * 3 + c] = (float)p->T_wc[r * 4 + c];
+      p->cw.r[r * 3 + c] = (float)Ti[r * 4 + c];
+    }
+    p->wc.t[r] = (float)p->T_wc[r * 4 + 3];
+    p->cw.t[r] = (float)Ti[r * 4 + 3];
+  }
+}
+
+// Euler-ZYX increment R = Rz(g) Ry(b) Rx(a) with translation t (same element
+// formulas as the reference's incremental transform).
+KF_HD void euler_zyx(double a, double b, double g, double tx, double ty, double tz, double* D) {
+  const double ca = cos(a), sa = sin(a), cb = cos(b), sb = sin(b), cg = cos(g), sg = sin(g);
+  D[0] = cg * cb;
+  D[1] = -sg * ca + (cg * sb) * sa;
+  D[2] = sg * sa + (cg * sb) * ca;
+  D[3] = tx;
+  D[4] = sg * cb;
+  D[5] = cg * ca + (sg * sb) * sa;
+  D[6] = -cg * sa + (sg * sb) * ca;
+  D[7] = ty;
+  D[8] = -sb;
+  D[9] = cb * sa;
+  D[10] = cb * ca;
+  D[11] = tz;
+  D[12] = 0.0;
+  D[13] = 0.0;
+  D[14] = 0.0;
+  D[15] = 1.0;
+}
+
+// Solves the 6x6 SPD system A x = g (A given by its 21 upper-triangle entries,
+// row-major i <= j) by Cholesky.  Returns false if a pivot is not above
+// rel_eps * max diagonal (or NaN).
+KF_HD bool cholesky_solve6(const double* upper21, const double* g, double rel_eps, double* x) {
+  double A[6][6];
+  int k = 0;
+  double dmax = 0.0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j) {
+      A[i][j] = upper21[k];
+      A[j][i] = upper21[k];
+      ++k;
+    }
+  for (int i = 0; i < 6; ++i) dmax = A[i][i] > dmax ? A[i][i] : dmax;
+  const double thr = rel_eps * dmax;
+  double L[6][6];
+  for (int j = 0; j < 6; ++j) {
+    double d = A[j][j];
+    for (int m = 0; m < j; ++m) d = d - L[j][m] * L[j][m];
+    if (!(d > thr)) return false;
+    const double ljj = sqrt(d);
+    L[j][j] = ljj;
+    for (int i = j + 1; i < 6; ++i) {
+      double s = A[i][j];
+      for (int m = 0; m < j; ++m) s = s - L[i][m] * L[j][m];
+      L[i][j] = s / ljj;
+    }
+  }
+  double y[6];
+  for (int i = 0; i < 6; ++i) {
+    double s = g[i];
+    for (int m = 0; m < i; ++m) s = s - L[i][m] * y[m];
+    y[i] = s / L[i][i];
+  }
+  for (int i = 5; i >= 0; --i) {
+    double s = y[i];
+    for (int m = i + 1; m < 6; ++m) s = s - L[m][i] * x[m];
+    x[i] = s / L[i][i];
+  }
+  return true;
+}
+
+}  // namespace kf

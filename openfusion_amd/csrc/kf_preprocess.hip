@@ -1,0 +1,231 @@
+// kf_preprocess.hip — depth conditioning for one frame on gfx950.
+//
+// Replaces d_convertDepthToWorldVertices / d_convertDepthToWorldNormals
+// (Source/src/CameraSupport/CameraHandler.cu:65-132) and adds the bilateral
+// filter and 3-level pyramid the reference only mentions (CameraHandler.cu:6).
+// Maps are kept in the camera frame; the tracker applies the pose itself.
+//
+//   depth_to_l0   : uint16 -> metres, range gate, raylen = |K^-1 [x,y,1]| * d,
+//                   bilateral filter (LDS tile with halo)            [per L0 pixel]
+//   pyr_down      : 2x2 block average gated by |d - centre| < gate   [per coarse pixel]
+//   vertex_normal : v = d * K^-1 [x,y,1];  n = normalize((v(x,y+1)-v) x (v(x+1,y)-v))
+//                   (points towards the camera), all levels in one launch
+#include "kf_device.hpp"
+#include "kf_internal.hpp"
+
+namespace kf {
+namespace {
+
+constexpr int kTile = 16;
+constexpr int kMaxRadius = 4;
+constexpr int kHalo = kTile + 2 * kMaxRadius;
+
+struct PreArgs {
+  const uint16_t* depth;
+  float* raylen;
+  float* out;  // filtered level-0 depth
+  int W, H;
+  float fx, fy, cx, cy;
+  float scale, dmin, dmax;
+  int radius;
+  float inv2ss, inv2sr;  // 0.5/sigma_s^2, 0.5/sigma_r^2
+};
+
+__global__ __launch_bounds__(256) void depth_to_l0(PreArgs a) {
+  __shared__ float tile[kHalo * kHalo];
+  const int R = a.radius;
+  const int span = kTile + 2 * R;
+  const int bx = blockIdx.x * kTile, by = blockIdx.y * kTile;
+  const int tid = threadIdx.y * kTile + threadIdx.x;
+  for (int i = tid; i < span * span; i += kTile * kTile) {
+    int ty = i / span, tx = i - ty * span;
+    int gx = bx + tx - R, gy = by + ty - R;
+    float d = 0.0f;
+    if (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H) {
+      d = (float)a.depth[gy * a.W + gx] * a.scale;
+      if (!(d >= a.dmin && d <= a.dmax)) d = 0.0f;
+    }
+    tile[ty * span + tx] = d;
+  }
+  __syncthreads();
+  const int x = bx + threadIdx.x, y = by + threadIdx.y;
+  if (x >= a.W || y >= a.H) return;
+  const float dc = tile[(threadIdx.y + R) * span + threadIdx.x + R];
+  const int idx = y * a.W + x;
+  if (dc == 0.0f) {
+    a.raylen[idx] = 0.0f;
+    a.out[idx] = 0.0f;
+    return;
+  }
+  const float xn = ((float)x - a.cx) / a.fx;
+  const float yn = ((float)y - a.cy) / a.fy;
+  const float lam = sqrtf((xn * xn + yn * yn) + 1.0f);
+  a.raylen[idx] = dc * lam;
+
+  float sw = 0.0f, swd = 0.0f;
+  for (int dy = -R; dy <= R; ++dy) {
+    const int qy = y + dy;
+    if (qy < 0 || qy >= a.H) continue;
+    for (int dx = -R; dx <= R; ++dx) {
+      const int qx = x + dx;
+      if (qx < 0 || qx >= a.W) continue;
+      const float dq = tile[(threadIdx.y + R + dy) * span + threadIdx.x + R + dx];
+      if (dq == 0.0f) continue;
+      const float diff = dq - dc;
+      const float arg = (float)(dx * dx + dy * dy) * a.inv2ss + (diff * diff) * a.inv2sr;
+      const float w = exp_neg(arg);
+      sw = sw + w;
+      swd = swd + w * dq;
+    }
+  }
+  a.out[idx] = swd / sw;
+}
+
+__global__ __launch_bounds__(256) void pyr_down(const float* in, int Wi, int Hi, float* out, int Wo,
+                                                int Ho, float gate) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= Wo || y >= Ho) return;
+  const float c = in[(2 * y) * Wi + 2 * x];
+  float r = 0.0f;
+  if (c != 0.0f) {
+    float s = 0.0f, n = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int sx = 2 * x + i, sy = 2 * y + j;
+        if (sx >= Wi || sy >= Hi) continue;
+        const float d = in[sy * Wi + sx];
+        if (d > 0.0f && fabsf(d - c) < gate) {
+          s = s + d;
+          n = n + 1.0f;
+        }
+      }
+    }
+    r = s / n;
+  }
+  out[y * Wo + x] = r;
+}
+
+struct LevelMapArgs {
+  const float* depth;
+  float *vx, *vy, *vz, *nx, *ny, *nz;
+  int W, H;
+  float fx, fy, cx, cy;
+  int block_begin;  // first linear block of this level
+  int blocks_x;
+};
+struct MapArgs {
+  LevelMapArgs lv[KF_MAX_LEVELS];
+  int levels;
+};
+
+__device__ __forceinline__ f3 backproject(const LevelMapArgs& a, int x, int y) {
+  const float d = a.depth[y * a.W + x];
+  if (d == 0.0f) return f3{nanf_(), nanf_(), nanf_()};
+  const float xn = ((float)x - a.cx) / a.fx;
+  const float yn = ((float)y - a.cy) / a.fy;
+  return f3{xn * d, yn * d, d};
+}
+
+__global__ __launch_bounds__(256) void vertex_normal(MapArgs m) {
+  int b = blockIdx.x;
+  int l = 0;
+  for (int i = 1; i < m.levels; ++i)
+    if (b >= m.lv[i].block_begin) l = i;
+  const LevelMapArgs& a = m.lv[l];
+  b -= a.block_begin;
+  const int by = b / a.blocks_x, bx = b - by * a.blocks_x;
+  const int x = bx * kTile + (threadIdx.x & (kTile - 1));
+  const int y = by * kTile + (threadIdx.x >> 4);
+  if (x >= a.W || y >= a.H) return;
+  const int idx = y * a.W + x;
+  const f3 v = backproject(a, x, y);
+  a.vx[idx] = v.x;
+  a.vy[idx] = v.y;
+  a.vz[idx] = v.z;
+  f3 n = f3{nanf_(), nanf_(), nanf_()};
+  if (v.z == v.z && x + 1 < a.W && y + 1 < a.H) {
+    const f3 vr = backproject(a, x + 1, y);
+    const f3 vd = backproject(a, x, y + 1);
+    if (vr.z == vr.z && vd.z == vd.z) {
+      const f3 c = cross(sub(vd, v), sub(vr, v));
+      const float len = sqrtf(dot(c, c));
+      if (len > 0.0f) {
+        const float inv = 1.0f / len;
+        n = f3{c.x * inv, c.y * inv, c.z * inv};
+      }
+    }
+  }
+  a.nx[idx] = n.x;
+  a.ny[idx] = n.y;
+  a.nz[idx] = n.z;
+}
+
+}  // namespace
+
+kf_status launch_preprocess(const kf_intrinsics& k0, const kf_preprocess_params& p,
+                            const uint16_t* d_depth, kf_frame_maps& live, hipStream_t s) {
+  KF_CHECK_ARG(d_depth != nullptr, "depth is NULL");
+  KF_CHECK_ARG(live.levels >= 1 && live.levels <= KF_MAX_LEVELS, "levels must be 1..3");
+  KF_CHECK_ARG(p.bilateral_radius >= 0 && p.bilateral_radius <= kMaxRadius, "bilateral_radius must be 0..4");
+  KF_CHECK_ARG(p.sigma_space_px > 0.0f && p.sigma_range_m > 0.0f, "sigmas must be positive");
+  KF_CHECK_ARG(live.raylen != nullptr, "live.raylen is NULL");
+  for (int l = 0; l < live.levels; ++l) {
+    const kf_level_maps& m = live.level[l];
+    kf_intrinsics kl = kf_intrinsics_level(&k0, l);
+    KF_CHECK_ARG(m.width == kl.width && m.height == kl.height, "map level size mismatch");
+    KF_CHECK_ARG(m.depth && m.vx && m.vy && m.vz && m.nx && m.ny && m.nz, "map plane is NULL");
+  }
+  PreArgs a;
+  a.depth = d_depth;
+  a.raylen = live.raylen;
+  a.out = live.level[0].depth;
+  a.W = k0.width;
+  a.H = k0.height;
+  a.fx = k0.fx;
+  a.fy = k0.fy;
+  a.cx = k0.cx;
+  a.cy = k0.cy;
+  a.scale = p.depth_scale;
+  a.dmin = p.min_depth_m;
+  a.dmax = p.max_depth_m;
+  a.radius = p.bilateral_radius;
+  a.inv2ss = 0.5f / (p.sigma_space_px * p.sigma_space_px);
+  a.inv2sr = 0.5f / (p.sigma_range_m * p.sigma_range_m);
+  dim3 blk(kTile, kTile);
+  dim3 grd(div_up(k0.width, kTile), div_up(k0.height, kTile));
+  hipLaunchKernelGGL(depth_to_l0, grd, blk, 0, s, a);
+  KF_TRY(check_launch("depth_to_l0"));
+
+  for (int l = 1; l < live.levels; ++l) {
+    const kf_level_maps& fi = live.level[l - 1];
+    const kf_level_maps& co = live.level[l];
+    dim3 g2(div_up(co.width, kTile), div_up(co.height, kTile));
+    hipLaunchKernelGGL(pyr_down, g2, blk, 0, s, (const float*)fi.depth, fi.width, fi.height, co.depth,
+                       co.width, co.height, p.pyramid_gate_m);
+    KF_TRY(check_launch("pyr_down"));
+  }
+
+  MapArgs m;
+  int blocks = 0;
+  m.levels = live.levels;
+  for (int l = 0; l < live.levels; ++l) {
+    kf_intrinsics kl = kf_intrinsics_level(&k0, l);
+    LevelMapArgs& la = m.lv[l];
+    const kf_level_maps& lm = live.level[l];
+    la.depth = lm.depth;
+    la.vx = lm.vx; la.vy = lm.vy; la.vz = lm.vz;
+    la.nx = lm.nx; la.ny = lm.ny; la.nz = lm.nz;
+    la.W = kl.width; la.H = kl.height;
+    la.fx = kl.fx; la.fy = kl.fy; la.cx = kl.cx; la.cy = kl.cy;
+    la.blocks_x = div_up(kl.width, kTile);
+    la.block_begin = blocks;
+    blocks += la.blocks_x * div_up(kl.height, kTile);
+  }
+  hipLaunchKernelGGL(vertex_normal, dim3(blocks), dim3(256), 0, s, m);
+  return check_launch("vertex_normal");
+}
+
+}  // namespace kf

@@ -1,0 +1,654 @@
+"""ctypes binding of libkinfu's C ABI (include/kinfu.h) and Python mirrors of the
+reference's reconstruction objects.
+
+The mirrors keep the reference's class and method names so that a driver written
+against OpenFusion's API reads the same:
+
+* ``General.Instance().InitVoxelArray`` / ``UpdateTransforms``  (Source/include/General.h:6-20)
+* ``CameraHandler.updateCameraData``                              (Source/include/CameraHandler.h:24)
+* ``VolumeIntegrator(params).Integrate``                          (Source/include/VolumeIntegrator.h:9-17)
+* ``Raycaster(params).Raycast``                                   (Source/include/Raycaster.h:6-13)
+* ``ICP(params).Iterate_ICP`` / ``FindCorresponding``             (Source/include/ICP.h:8-70)
+
+All work runs in libkinfu's HIP kernels; there is no CPU fallback.  Loading
+fails loudly if the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libkinfu.so")
+
+KF_OK = 0
+KF_VOXEL_F32 = 0
+KF_VOXEL_F16 = 1
+MAX_LEVELS = 3
+
+# ---------------------------------------------------------------------------
+# Structs (field order and types mirror include/kinfu.h).
+
+
+class Intrinsics(C.Structure):
+    _fields_ = [("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float),
+                ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class VolumeDesc(C.Structure):
+    _fields_ = [("n", C.c_int32), ("extent_m", C.c_float), ("origin_m", C.c_float * 3),
+                ("trunc_m", C.c_float), ("max_weight", C.c_float), ("format", C.c_int32)]
+
+
+class PoseF(C.Structure):
+    _fields_ = [("r", C.c_float * 9), ("t", C.c_float * 3)]
+
+
+class Pose(C.Structure):
+    _fields_ = [("T_wc", C.c_double * 16), ("wc", PoseF), ("cw", PoseF)]
+
+
+_FP = C.POINTER(C.c_float)
+
+
+class LevelMaps(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("depth", _FP),
+                ("vx", _FP), ("vy", _FP), ("vz", _FP), ("nx", _FP), ("ny", _FP), ("nz", _FP)]
+
+
+class FrameMaps(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("raylen", _FP), ("level", LevelMaps * MAX_LEVELS)]
+
+
+class PreprocessParams(C.Structure):
+    _fields_ = [("depth_scale", C.c_float), ("min_depth_m", C.c_float), ("max_depth_m", C.c_float),
+                ("bilateral_radius", C.c_int32), ("sigma_space_px", C.c_float),
+                ("sigma_range_m", C.c_float), ("pyramid_gate_m", C.c_float)]
+
+
+class IntegrateParams(C.Structure):
+    _fields_ = [("min_depth_m", C.c_float), ("max_depth_m", C.c_float)]
+
+
+class RaycastParams(C.Structure):
+    _fields_ = [("min_depth_m", C.c_float), ("step_far", C.c_float), ("step_near", C.c_float),
+                ("max_steps", C.c_int32)]
+
+
+class IcpParams(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("iterations", C.c_int32 * MAX_LEVELS),
+                ("dist_thresh_m", C.c_float), ("normal_thresh_deg", C.c_float),
+                ("compensated", C.c_int32)]
+
+
+class IcpStats(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("inliers", C.c_int32),
+                ("candidates", C.c_int32), ("residual_rms", C.c_double),
+                ("level_residual", C.c_double * MAX_LEVELS), ("level_inliers", C.c_int32 * MAX_LEVELS),
+                ("skip", C.c_int32), ("pad_", C.c_int32)]
+
+
+class PipelineDesc(C.Structure):
+    _fields_ = [("intr", Intrinsics), ("volume", VolumeDesc), ("pre", PreprocessParams),
+                ("integ", IntegrateParams), ("ray", RaycastParams), ("icp", IcpParams),
+                ("icp_prior", IcpParams), ("fallback_residual_ratio", C.c_float),
+                ("fallback_min_inlier_frac", C.c_float), ("use_graph", C.c_int32)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("frame", C.c_int32), ("used_prior", C.c_int32), ("fell_back", C.c_int32),
+                ("pad_", C.c_int32), ("icp", IcpStats), ("icp_prior", IcpStats),
+                ("updated_voxels", C.c_uint64), ("raycast_steps", C.c_uint64)]
+
+
+STAGES = 5  # preprocess, guess+icp, column spans, sweep kernel, raycast
+
+
+# ---------------------------------------------------------------------------
+# Library loading and prototypes.
+
+_VP = C.c_void_p
+_PROTOS = {
+    "kf_last_error": (C.c_char_p, []),
+    "kf_intrinsics_level": (Intrinsics, [C.POINTER(Intrinsics), C.c_int32]),
+    "kf_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
+    "kf_device_malloc": (C.c_int, [C.POINTER(_VP), C.c_size_t]),
+    "kf_device_free": (C.c_int, [_VP]),
+    "kf_host_malloc_pinned": (C.c_int, [C.POINTER(_VP), C.c_size_t]),
+    "kf_host_free_pinned": (C.c_int, [_VP]),
+    "kf_memcpy_htod": (C.c_int, [_VP, _VP, C.c_size_t, _VP]),
+    "kf_memcpy_dtoh": (C.c_int, [_VP, _VP, C.c_size_t, _VP]),
+    "kf_memcpy_dtod": (C.c_int, [_VP, _VP, C.c_size_t, _VP]),
+    "kf_memset_d8": (C.c_int, [_VP, C.c_int, C.c_size_t, _VP]),
+    "kf_stream_create": (C.c_int, [C.POINTER(_VP)]),
+    "kf_stream_destroy": (C.c_int, [_VP]),
+    "kf_stream_synchronize": (C.c_int, [_VP]),
+    "kf_device_synchronize": (C.c_int, []),
+    "kf_event_create": (C.c_int, [C.POINTER(_VP)]),
+    "kf_event_destroy": (C.c_int, [_VP]),
+    "kf_event_record": (C.c_int, [_VP, _VP]),
+    "kf_event_elapsed_ms": (C.c_int, [_VP, _VP, C.POINTER(C.c_float)]),
+    "kf_maps_alloc": (C.c_int, [C.POINTER(Intrinsics), C.c_int32, C.c_int32, C.POINTER(FrameMaps)]),
+    "kf_maps_free": (C.c_int, [C.POINTER(FrameMaps)]),
+    "kf_pose_upload": (C.c_int, [_VP, C.POINTER(C.c_double), _VP]),
+    "kf_pose_download": (C.c_int, [C.POINTER(C.c_double), _VP, _VP]),
+    "kf_pose_finish_host": (None, [C.POINTER(Pose)]),
+    "kf_volume_create": (C.c_int, [C.POINTER(VolumeDesc), C.POINTER(_VP)]),
+    "kf_volume_destroy": (C.c_int, [_VP]),
+    "kf_volume_reset": (C.c_int, [_VP, _VP]),
+    "kf_volume_get_desc": (C.c_int, [_VP, C.POINTER(VolumeDesc)]),
+    "kf_volume_data": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(C.c_size_t)]),
+    "kf_volume_stats": (C.c_int, [_VP, _VP, _VP]),
+    "kf_preprocess": (C.c_int, [C.POINTER(Intrinsics), C.POINTER(PreprocessParams), _VP,
+                                C.POINTER(FrameMaps), _VP]),
+    "kf_integrate": (C.c_int, [_VP, C.POINTER(Intrinsics), C.POINTER(IntegrateParams), _VP, _VP, _VP, _VP,
+                               _VP]),
+    "kf_raycast": (C.c_int, [_VP, C.POINTER(Intrinsics), C.POINTER(RaycastParams), _VP,
+                             C.POINTER(FrameMaps), _VP, _VP]),
+    "kf_tracker_create": (C.c_int, [C.POINTER(Intrinsics), C.POINTER(IcpParams), C.POINTER(_VP)]),
+    "kf_tracker_destroy": (C.c_int, [_VP]),
+    "kf_icp_track": (C.c_int, [_VP, C.POINTER(FrameMaps), C.POINTER(FrameMaps), _VP, _VP, _VP, _VP]),
+    "kf_icp_reduce": (C.c_int, [_VP, C.c_int32, C.POINTER(FrameMaps), C.POINTER(FrameMaps), _VP, _VP, _VP,
+                                _VP]),
+    "kf_pose_constant_velocity": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "kf_pipeline_default_desc": (C.c_int, [C.POINTER(PipelineDesc)]),
+    "kf_pipeline_create": (C.c_int, [C.POINTER(PipelineDesc), _VP, C.POINTER(_VP)]),
+    "kf_pipeline_destroy": (C.c_int, [_VP]),
+    "kf_pipeline_reset": (C.c_int, [_VP, C.POINTER(C.c_double)]),
+    "kf_pipeline_process": (C.c_int, [_VP, _VP]),
+    "kf_pipeline_get_pose": (C.c_int, [_VP, C.POINTER(C.c_double)]),
+    "kf_pipeline_get_stats": (C.c_int, [_VP, C.POINTER(FrameStats)]),
+    "kf_pipeline_get_trajectory": (C.c_int, [_VP, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32)]),
+    "kf_pipeline_get_volume": (C.c_int, [_VP, C.POINTER(_VP)]),
+    "kf_pipeline_get_maps": (C.c_int, [_VP, C.POINTER(FrameMaps), C.POINTER(FrameMaps)]),
+    "kf_pipeline_get_stream": (C.c_int, [_VP, C.POINTER(_VP)]),
+    "kf_pipeline_set_stage_timing": (C.c_int, [_VP, C.c_int32]),
+    "kf_pipeline_get_stage_ms": (C.c_int, [_VP, C.POINTER(C.c_float)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded libkinfu (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m openfusion_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _PROTOS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def extra_protos(table: dict) -> None:
+    """Registers prototypes of entry points declared outside kinfu.h (same library)."""
+    L = lib()
+    for name, (res, args) in table.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+class KinfuError(RuntimeError):
+    pass
+
+
+def check(status: int, what: str = "") -> None:
+    if status != KF_OK:
+        msg = lib().kf_last_error().decode(errors="replace")
+        raise KinfuError(f"{what}: status {status}: {msg}")
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    lib().kf_device_count(C.byref(n))
+    return int(n.value)
+
+
+# ---------------------------------------------------------------------------
+# Device memory.
+
+
+class DeviceBuffer:
+    """A hipMalloc'd byte buffer owned by Python."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = _VP()
+        check(lib().kf_device_malloc(C.byref(p), max(self.nbytes, 1)), "kf_device_malloc")
+        self.ptr = p.value
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().kf_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def upload(self, a: np.ndarray, stream=None) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().kf_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, stream), "htod")
+        check(lib().kf_stream_synchronize(stream), "sync")
+
+    def download(self, dtype, count: int | None = None, stream=None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = self.nbytes // dtype.itemsize
+        out = np.empty(count, dtype=dtype)
+        check(lib().kf_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes, stream), "dtoh")
+        return out
+
+    def zero(self, stream=None) -> None:
+        check(lib().kf_memset_d8(self.ptr, 0, self.nbytes, stream), "memset")
+        check(lib().kf_stream_synchronize(stream), "sync")
+
+
+def upload(a: np.ndarray) -> DeviceBuffer:
+    b = DeviceBuffer(np.ascontiguousarray(a).nbytes)
+    b.upload(a)
+    return b
+
+
+def synchronize() -> None:
+    check(lib().kf_device_synchronize(), "kf_device_synchronize")
+
+
+def download_ptr(ptr: int, dtype, count: int) -> np.ndarray:
+    out = np.empty(count, dtype=np.dtype(dtype))
+    check(lib().kf_memcpy_dtoh(out.ctypes.data, ptr, out.nbytes, None), "dtoh")
+    return out
+
+
+def upload_ptr(ptr: int, a: np.ndarray) -> None:
+    a = np.ascontiguousarray(a)
+    check(lib().kf_memcpy_htod(ptr, a.ctypes.data, a.nbytes, None), "htod")
+    check(lib().kf_stream_synchronize(None), "sync")
+
+
+# ---------------------------------------------------------------------------
+# Poses.
+
+
+class DevicePose:
+    """A kf_pose in device memory (T_wc authoritative, FP32 copies derived)."""
+
+    def __init__(self, T: np.ndarray | None = None):
+        self.buf = DeviceBuffer(C.sizeof(Pose))
+        self.ptr = self.buf.ptr
+        if T is not None:
+            self.set(T)
+
+    def set(self, T: np.ndarray) -> None:
+        T = np.ascontiguousarray(T, dtype=np.float64).reshape(16)
+        check(lib().kf_pose_upload(self.ptr, T.ctypes.data_as(C.POINTER(C.c_double)), None), "pose_upload")
+
+    def get(self) -> np.ndarray:
+        T = np.empty(16, dtype=np.float64)
+        check(lib().kf_pose_download(T.ctypes.data_as(C.POINTER(C.c_double)), self.ptr, None), "pose_download")
+        return T.reshape(4, 4)
+
+    def raw(self) -> Pose:
+        p = Pose()
+        check(lib().kf_memcpy_dtoh(C.addressof(p), self.ptr, C.sizeof(Pose), None), "dtoh")
+        return p
+
+
+def pose_fields_host(T: np.ndarray) -> Pose:
+    p = Pose()
+    for i, v in enumerate(np.asarray(T, dtype=np.float64).reshape(16)):
+        p.T_wc[i] = v
+    lib().kf_pose_finish_host(C.byref(p))
+    return p
+
+
+# ---------------------------------------------------------------------------
+# Maps.
+
+
+def intrinsics_level(k0: Intrinsics, level: int) -> Intrinsics:
+    return lib().kf_intrinsics_level(C.byref(k0), level)
+
+
+class DeviceMaps:
+    """kf_frame_maps allocated by kf_maps_alloc (live: with depth + raylen)."""
+
+    def __init__(self, k0: Intrinsics, levels: int = 3, with_depth: bool = True):
+        self.k0 = k0
+        self.maps = FrameMaps()
+        self.with_depth = with_depth
+        check(lib().kf_maps_alloc(C.byref(k0), levels, int(with_depth), C.byref(self.maps)), "kf_maps_alloc")
+        self._owned = True
+
+    @classmethod
+    def wrap(cls, maps: FrameMaps, k0: Intrinsics) -> "DeviceMaps":
+        o = cls.__new__(cls)
+        o.k0 = k0
+        o.maps = maps
+        o.with_depth = bool(maps.raylen)
+        o._owned = False
+        return o
+
+    def free(self) -> None:
+        if getattr(self, "_owned", False) and self.maps.level[0].vx:
+            lib().kf_maps_free(C.byref(self.maps))
+            self._owned = False
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _addr(p) -> int:
+        return C.cast(p, C.c_void_p).value
+
+    def plane(self, level: int, name: str) -> np.ndarray:
+        lv = self.maps.level[level]
+        n = lv.width * lv.height
+        return download_ptr(self._addr(getattr(lv, name)), np.float32, n).reshape(lv.height, lv.width)
+
+    def raylen(self) -> np.ndarray:
+        lv = self.maps.level[0]
+        return download_ptr(self._addr(self.maps.raylen), np.float32, lv.width * lv.height).reshape(
+            lv.height, lv.width)
+
+    def vertex(self, level: int = 0) -> np.ndarray:
+        return np.stack([self.plane(level, "vx"), self.plane(level, "vy"), self.plane(level, "vz")], -1)
+
+    def normal(self, level: int = 0) -> np.ndarray:
+        return np.stack([self.plane(level, "nx"), self.plane(level, "ny"), self.plane(level, "nz")], -1)
+
+    def set_plane(self, level: int, name: str, a: np.ndarray) -> None:
+        upload_ptr(self._addr(getattr(self.maps.level[level], name)), np.asarray(a, np.float32))
+
+
+# ---------------------------------------------------------------------------
+# Volume.
+
+
+class Volume:
+    def __init__(self, n: int, extent_m: float, trunc_m: float, max_weight: float = 128.0,
+                 fmt: int = KF_VOXEL_F32, origin=None):
+        d = VolumeDesc()
+        d.n = n
+        d.extent_m = extent_m
+        o = origin if origin is not None else (-extent_m / 2,) * 3
+        for i in range(3):
+            d.origin_m[i] = o[i]
+        d.trunc_m = trunc_m
+        d.max_weight = max_weight
+        d.format = fmt
+        self.desc = d
+        h = _VP()
+        check(lib().kf_volume_create(C.byref(d), C.byref(h)), "kf_volume_create")
+        self.handle = h.value
+        self._owned = True
+
+    @classmethod
+    def wrap(cls, handle: int) -> "Volume":
+        o = cls.__new__(cls)
+        o.handle = handle
+        d = VolumeDesc()
+        check(lib().kf_volume_get_desc(handle, C.byref(d)), "kf_volume_get_desc")
+        o.desc = d
+        o._owned = False
+        return o
+
+    def __del__(self):
+        try:
+            if getattr(self, "_owned", False) and self.handle:
+                lib().kf_volume_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def data_ptr(self) -> tuple[int, int]:
+        p = _VP()
+        nb = C.c_size_t()
+        check(lib().kf_volume_data(self.handle, C.byref(p), C.byref(nb)), "kf_volume_data")
+        return p.value, nb.value
+
+    def download_raw(self) -> np.ndarray:
+        p, nb = self.data_ptr()
+        return download_ptr(p, np.uint8, nb)
+
+    def upload_raw(self, raw: np.ndarray) -> None:
+        p, nb = self.data_ptr()
+        assert raw.nbytes == nb
+        upload_ptr(p, raw)
+
+    def stats(self) -> np.ndarray:
+        """[observed voxels, sum w, sum f (observed), observed voxels with f < 0]."""
+        out = DeviceBuffer(4 * 8)
+        check(lib().kf_volume_stats(self.handle, out.ptr, None), "kf_volume_stats")
+        return out.download(np.float64, 4)
+
+    def tsdf_weight(self) -> tuple[np.ndarray, np.ndarray]:
+        """(tsdf/T, weight) as float32 arrays of shape (n, n, n), index [x, y, z]."""
+        n = self.desc.n
+        raw = self.download_raw()
+        if self.desc.format == KF_VOXEL_F16:
+            w32 = raw.view(np.uint32)
+            f = (w32 & 0xFFFF).astype(np.uint16).view(np.float16).astype(np.float32)
+            w = (w32 >> 16).astype(np.float32)
+        else:
+            fw = raw.view(np.float32).reshape(-1, 2)
+            f, w = fw[:, 0].copy(), fw[:, 1].copy()
+        return f.reshape(n, n, n), w.reshape(n, n, n)
+
+
+# ---------------------------------------------------------------------------
+# Parameter helpers.
+
+
+def default_pipeline_desc() -> PipelineDesc:
+    d = PipelineDesc()
+    check(lib().kf_pipeline_default_desc(C.byref(d)), "kf_pipeline_default_desc")
+    return d
+
+
+def _ptr(x) -> int | None:
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.ptr
+
+
+# ---------------------------------------------------------------------------
+# Mirrors of the reference objects.
+
+
+class General:
+    """Mirror of General (Source/include/General.h:6-20): volume init and pose bookkeeping."""
+
+    _instance = None
+
+    @classmethod
+    def Instance(cls) -> "General":
+        if cls._instance is None:
+            cls._instance = cls()
+        return cls._instance
+
+    def InitVoxelArray(self, volume: Volume, stream=None) -> None:
+        check(lib().kf_volume_reset(volume.handle, stream), "InitVoxelArray")
+        check(lib().kf_stream_synchronize(stream), "sync")
+
+    def UpdateTransforms(self, pose: DevicePose, T_wc: np.ndarray) -> None:
+        """Sets the device pose (T_wc and its derived inverse / FP32 copies)."""
+        pose.set(T_wc)
+
+
+class CameraHandler:
+    """Mirror of CameraHandler::updateCameraData (Source/src/CameraSupport/CameraHandler.cu:134-143)."""
+
+    @staticmethod
+    def updateCameraData(k0: Intrinsics, params: PreprocessParams, live: DeviceMaps, d_depth,
+                         stream=None) -> None:
+        check(lib().kf_preprocess(C.byref(k0), C.byref(params), _ptr(d_depth), C.byref(live.maps), stream),
+              "updateCameraData")
+
+
+class VolumeIntegrator:
+    """Mirror of VolumeIntegrator (Source/include/VolumeIntegrator.h:9-17)."""
+
+    def __init__(self, k0: Intrinsics, params: IntegrateParams):
+        self.k0 = k0
+        self.params = params
+
+    def Integrate(self, volume: Volume, live: DeviceMaps, pose: DevicePose, updated=None, mask=None,
+                  stream=None) -> None:
+        check(lib().kf_integrate(volume.handle, C.byref(self.k0), C.byref(self.params),
+                                 C.cast(live.maps.raylen, C.c_void_p).value, pose.ptr, _ptr(updated),
+                                 _ptr(mask), stream), "Integrate")
+
+
+class Raycaster:
+    """Mirror of Raycaster (Source/include/Raycaster.h:6-13)."""
+
+    def __init__(self, k0: Intrinsics, params: RaycastParams):
+        self.k0 = k0
+        self.params = params
+
+    def Raycast(self, volume: Volume, pose: DevicePose, model: DeviceMaps, steps=None, stream=None) -> None:
+        check(lib().kf_raycast(volume.handle, C.byref(self.k0), C.byref(self.params), pose.ptr,
+                               C.byref(model.maps), _ptr(steps), stream), "Raycast")
+
+
+class ICP:
+    """Mirror of ICP (Source/include/ICP.h:8-70); the pose is refined in place."""
+
+    def __init__(self, k0: Intrinsics, params: IcpParams):
+        self.k0 = k0
+        self.params = params
+        h = _VP()
+        check(lib().kf_tracker_create(C.byref(k0), C.byref(params), C.byref(h)), "kf_tracker_create")
+        self.handle = h.value
+        self.stats = DeviceBuffer(C.sizeof(IcpStats))
+        self.stats.zero()
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().kf_tracker_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def Iterate_ICP(self, live: DeviceMaps, model: DeviceMaps, pose: DevicePose, guess: DevicePose | None = None,
+                    stream=None) -> IcpStats:
+        g = guess if guess is not None else pose
+        check(lib().kf_icp_track(self.handle, C.byref(live.maps), C.byref(model.maps), g.ptr, pose.ptr,
+                                 self.stats.ptr, stream), "Iterate_ICP")
+        st = IcpStats()
+        check(lib().kf_memcpy_dtoh(C.addressof(st), self.stats.ptr, C.sizeof(IcpStats), stream), "dtoh")
+        return st
+
+    def FindCorresponding(self, level: int, live: DeviceMaps, model: DeviceMaps, pose: DevicePose,
+                          with_mask: bool = False, stream=None):
+        """One association + reduction pass: returns (30 FP64 sums, inlier mask or None)."""
+        sums = DeviceBuffer(30 * 8)
+        lv = live.maps.level[level]
+        mask = DeviceBuffer(lv.width * lv.height) if with_mask else None
+        check(lib().kf_icp_reduce(self.handle, level, C.byref(live.maps), C.byref(model.maps), pose.ptr, sums.ptr,
+                                  _ptr(mask), stream), "FindCorresponding")
+        s = sums.download(np.float64, 30, stream)
+        m = mask.download(np.uint8, stream=stream).reshape(lv.height, lv.width) if mask else None
+        return s, m
+
+
+# ---------------------------------------------------------------------------
+# Pipeline.
+
+
+class Pipeline:
+    """kf_pipeline: preprocess -> track -> integrate -> raycast per frame (hipGraph replay)."""
+
+    def __init__(self, desc: PipelineDesc, stream=None):
+        self.desc = desc
+        h = _VP()
+        check(lib().kf_pipeline_create(C.byref(desc), stream, C.byref(h)), "kf_pipeline_create")
+        self.handle = h.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().kf_pipeline_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self, T0: np.ndarray) -> None:
+        T = np.ascontiguousarray(T0, dtype=np.float64).reshape(16)
+        check(lib().kf_pipeline_reset(self.handle, T.ctypes.data_as(C.POINTER(C.c_double))), "reset")
+
+    def process(self, d_depth) -> None:
+        check(lib().kf_pipeline_process(self.handle, _ptr(d_depth)), "process")
+
+    def pose(self) -> np.ndarray:
+        T = np.empty(16, dtype=np.float64)
+        check(lib().kf_pipeline_get_pose(self.handle, T.ctypes.data_as(C.POINTER(C.c_double))), "get_pose")
+        return T.reshape(4, 4)
+
+    def trajectory(self, max_frames: int = 8192) -> np.ndarray:
+        """(frames, 4, 4) T_wc of every frame since the last reset (device log)."""
+        out = np.zeros((max_frames, 16), dtype=np.float64)
+        n = C.c_int32(0)
+        check(lib().kf_pipeline_get_trajectory(self.handle, out.ctypes.data_as(C.POINTER(C.c_double)), max_frames,
+                                               C.byref(n)), "get_trajectory")
+        return out[: min(n.value, max_frames)].reshape(-1, 4, 4)
+
+    def stats(self) -> FrameStats:
+        s = FrameStats()
+        check(lib().kf_pipeline_get_stats(self.handle, C.byref(s)), "get_stats")
+        return s
+
+    def volume(self) -> Volume:
+        h = _VP()
+        check(lib().kf_pipeline_get_volume(self.handle, C.byref(h)), "get_volume")
+        return Volume.wrap(h.value)
+
+    def maps(self) -> tuple[DeviceMaps, DeviceMaps]:
+        live, model = FrameMaps(), FrameMaps()
+        check(lib().kf_pipeline_get_maps(self.handle, C.byref(live), C.byref(model)), "get_maps")
+        return DeviceMaps.wrap(live, self.desc.intr), DeviceMaps.wrap(model, self.desc.intr)
+
+    def stream(self):
+        s = _VP()
+        check(lib().kf_pipeline_get_stream(self.handle, C.byref(s)), "get_stream")
+        return s.value
+
+    def set_stage_timing(self, on: bool) -> None:
+        check(lib().kf_pipeline_set_stage_timing(self.handle, int(on)), "set_stage_timing")
+
+    def stage_ms(self) -> np.ndarray:
+        ms = (C.c_float * STAGES)()
+        check(lib().kf_pipeline_get_stage_ms(self.handle, ms), "get_stage_ms")
+        return np.array(ms[:], dtype=np.float64)
+
+
+def exported_symbols() -> list[str]:
+    """Names of every kf_* function declared in include/*.h."""
+    import re
+    inc = os.path.join(os.path.dirname(_PKG), "include")
+    names = []
+    for fn in sorted(os.listdir(inc)):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(inc, fn)).read()
+            names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_\s\*]*?\b(kf_[a-z0-9_]+)\s*\(", txt, re.M)
+    return sorted(set(names))

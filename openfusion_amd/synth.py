@@ -1,0 +1,211 @@
+"""Seeded synthetic depth streams (ctypes over libkfsynth, include/kf_synth.h) and the
+named workload presets.
+
+Presets (volume, truncation, scene, trajectory and seeds):
+
+* ``small``   — 256^3 FP32 over 3 m, T 0.03 m; 30 frames; 3x3x3 m room, 3 spheres
+  (r 0.2-0.5 m) + 2 boxes; arc of 2 cm / 1 deg per frame.
+* ``full``    — 512^3 FP32 over 4 m, T 0.02 m; 300 frames; 6-10 random primitives;
+  smooth random path <= 3 cm / 2 deg per frame.
+* ``full16``  — ``full`` with packed FP16 tsdf + u16 weight voxels.
+* ``prior``   — ``full16`` stream for the learned prior (training trajectories:
+  seeds 10000-10199, <= 6 cm / 4 deg per frame, 101 frames each).
+* ``stress``  — 768^3 FP16 over 6 m, T 0.03 m; 1000 frames; 6x3x6 m room with 20
+  primitives; bursts up to 6 cm / 5 deg per frame.
+
+All share K = (525, 525, 319.5, 239.5) at 640x480, depth range [0.3, 4] m,
+ICP 3 levels 10/5/4 (finest first) with 0.1 m / 20 deg gates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+
+import numpy as np
+
+from . import kinfu as kf
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+SYNTH_PATH = os.path.join(_PKG, "libkfsynth.so")
+
+KF_PRIM_SPHERE, KF_PRIM_BOX = 0, 1
+KF_TRAJ_ARC, KF_TRAJ_SPLINE, KF_TRAJ_BURST = 0, 1, 2
+MAX_PRIMS = 64
+
+
+class Primitive(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad_", C.c_int32), ("center", C.c_double * 3), ("radius", C.c_double),
+                ("half", C.c_double * 3)]
+
+
+class Scene(C.Structure):
+    _fields_ = [("room", C.c_double * 3), ("count", C.c_int32), ("pad_", C.c_int32),
+                ("prim", Primitive * MAX_PRIMS)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("room", C.c_double * 3), ("n_spheres", C.c_int32), ("n_boxes", C.c_int32),
+                ("n_random_min", C.c_int32), ("n_random_max", C.c_int32), ("sphere_radius", C.c_double * 2),
+                ("box_half", C.c_double * 2), ("clearance_m", C.c_double), ("seed", C.c_uint64)]
+
+
+class TrajectoryDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("frames", C.c_int32), ("max_trans_m", C.c_double),
+                ("max_rot_deg", C.c_double), ("room", C.c_double * 3), ("margin_m", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+_slib = None
+
+
+def slib() -> C.CDLL:
+    global _slib
+    if _slib is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise RuntimeError(f"{SYNTH_PATH} is missing: build it with `python -m openfusion_amd.build`")
+        L = C.CDLL(SYNTH_PATH)
+        D = C.POINTER(C.c_double)
+        L.kf_synth_trajectory.argtypes = [C.POINTER(TrajectoryDesc), D]
+        L.kf_synth_scene.argtypes = [C.POINTER(SceneDesc), D, C.c_int32, C.POINTER(Scene)]
+        L.kf_synth_render.argtypes = [C.POINTER(Scene), C.POINTER(kf.Intrinsics), D, C.c_uint64, C.c_int32,
+                                      C.c_double, C.c_double, C.c_void_p, C.c_void_p]
+        L.kf_synth_render_batch.argtypes = [C.POINTER(Scene), C.POINTER(kf.Intrinsics), D, C.c_int32, C.c_uint64,
+                                            C.c_double, C.c_double, C.c_void_p]
+        for f in (L.kf_synth_trajectory, L.kf_synth_scene, L.kf_synth_render, L.kf_synth_render_batch):
+            f.restype = C.c_int32
+        _slib = L
+    return _slib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@dataclasses.dataclass
+class Preset:
+    name: str
+    n: int
+    extent: float
+    trunc: float
+    fmt: int
+    frames: int
+    room: tuple
+    n_spheres: int = 0
+    n_boxes: int = 0
+    n_random: tuple = (0, 0)
+    traj: int = KF_TRAJ_SPLINE
+    max_trans: float = 0.03
+    max_rot: float = 2.0
+    scene_seed: int = 1001
+    traj_seed: int = 3001
+    noise_seed: int = 2001
+    margin: float = 0.7
+    clearance: float = 0.45
+    max_weight: float = 128.0
+    min_depth: float = 0.3
+    max_depth: float = 4.0
+
+
+PRESETS = {
+    "small": Preset("small", 256, 3.0, 0.03, kf.KF_VOXEL_F32, 30, (3.0, 3.0, 3.0), n_spheres=3, n_boxes=2,
+                    traj=KF_TRAJ_ARC, max_trans=0.02, max_rot=1.0, scene_seed=1000, traj_seed=3000,
+                    noise_seed=2000),
+    "full": Preset("full", 512, 4.0, 0.02, kf.KF_VOXEL_F32, 300, (3.0, 3.0, 3.0), n_random=(6, 10)),
+    "full16": Preset("full16", 512, 4.0, 0.02, kf.KF_VOXEL_F16, 300, (3.0, 3.0, 3.0), n_random=(6, 10)),
+    "stress": Preset("stress", 768, 6.0, 0.03, kf.KF_VOXEL_F16, 1000, (6.0, 3.0, 6.0), n_random=(20, 20),
+                     traj=KF_TRAJ_BURST, max_trans=0.06, max_rot=5.0, scene_seed=1004, traj_seed=3004,
+                     noise_seed=2004, margin=1.0),
+}
+
+
+def intrinsics() -> kf.Intrinsics:
+    return kf.Intrinsics(525.0, 525.0, 319.5, 239.5, 640, 480)
+
+
+def trajectory(kind: int, frames: int, max_trans: float, max_rot: float, room, margin: float,
+               seed: int) -> np.ndarray:
+    d = TrajectoryDesc()
+    d.kind, d.frames, d.max_trans_m, d.max_rot_deg = kind, frames, max_trans, max_rot
+    for i in range(3):
+        d.room[i] = room[i]
+    d.margin_m, d.seed = margin, seed
+    out = np.zeros((frames, 16), dtype=np.float64)
+    if slib().kf_synth_trajectory(C.byref(d), _dptr(out)) != 0:
+        raise ValueError("kf_synth_trajectory: invalid arguments")
+    return out.reshape(frames, 4, 4)
+
+
+def scene(room, poses: np.ndarray, n_spheres=0, n_boxes=0, n_random=(0, 0), seed=0, clearance=0.45,
+          sphere_radius=(0.2, 0.5), box_half=(0.1, 0.35)) -> Scene:
+    d = SceneDesc()
+    for i in range(3):
+        d.room[i] = room[i]
+    d.n_spheres, d.n_boxes = n_spheres, n_boxes
+    d.n_random_min, d.n_random_max = n_random
+    d.sphere_radius[0], d.sphere_radius[1] = sphere_radius
+    d.box_half[0], d.box_half[1] = box_half
+    d.clearance_m, d.seed = clearance, seed
+    s = Scene()
+    P = np.ascontiguousarray(poses.reshape(-1, 16), dtype=np.float64)
+    if slib().kf_synth_scene(C.byref(d), _dptr(P), P.shape[0], C.byref(s)) != 0:
+        raise ValueError("kf_synth_scene: invalid arguments")
+    return s
+
+
+def render(sc: Scene, k: kf.Intrinsics, T: np.ndarray, noise_seed: int, frame: int, noise_scale: float = 1.0,
+           max_range: float = 4.0, true_depth: bool = False):
+    W, H = k.width, k.height
+    mm = np.zeros(W * H, dtype=np.uint16)
+    tz = np.zeros(W * H, dtype=np.float32) if true_depth else None
+    T = np.ascontiguousarray(T, dtype=np.float64).reshape(16)
+    r = slib().kf_synth_render(C.byref(sc), C.byref(k), _dptr(T), noise_seed, frame, noise_scale, max_range,
+                               mm.ctypes.data, tz.ctypes.data if tz is not None else None)
+    if r != 0:
+        raise ValueError("kf_synth_render: invalid arguments")
+    mm = mm.reshape(H, W)
+    return (mm, tz.reshape(H, W)) if true_depth else mm
+
+
+def render_batch(sc: Scene, k: kf.Intrinsics, poses: np.ndarray, noise_seed: int, noise_scale: float = 1.0,
+                 max_range: float = 4.0) -> np.ndarray:
+    F = poses.shape[0]
+    out = np.zeros((F, k.height, k.width), dtype=np.uint16)
+    P = np.ascontiguousarray(poses.reshape(F, 16), dtype=np.float64)
+    if slib().kf_synth_render_batch(C.byref(sc), C.byref(k), _dptr(P), F, noise_seed, noise_scale, max_range,
+                                    out.ctypes.data) != 0:
+        raise ValueError("kf_synth_render_batch: invalid arguments")
+    return out
+
+
+@dataclasses.dataclass
+class Stream:
+    preset: Preset
+    poses: np.ndarray  # (F, 4, 4) ground truth T_wc
+    scene: Scene
+    depth: np.ndarray  # (F, H, W) uint16 mm
+
+
+def make_stream(preset: str | Preset, frames: int | None = None) -> Stream:
+    p = PRESETS[preset] if isinstance(preset, str) else preset
+    F = frames if frames is not None else p.frames
+    poses = trajectory(p.traj, F, p.max_trans, p.max_rot, p.room, p.margin, p.traj_seed)
+    sc = scene(p.room, poses, p.n_spheres, p.n_boxes, p.n_random, p.scene_seed, p.clearance)
+    depth = render_batch(sc, intrinsics(), poses, p.noise_seed, 1.0, p.max_depth)
+    return Stream(p, poses, sc, depth)
+
+
+def pipeline_desc(p: Preset | str) -> kf.PipelineDesc:
+    p = PRESETS[p] if isinstance(p, str) else p
+    d = kf.default_pipeline_desc()
+    d.intr = intrinsics()
+    d.volume.n = p.n
+    d.volume.extent_m = p.extent
+    for i in range(3):
+        d.volume.origin_m[i] = -p.extent / 2
+    d.volume.trunc_m = p.trunc
+    d.volume.max_weight = p.max_weight
+    d.volume.format = p.fmt
+    d.pre.min_depth_m = d.integ.min_depth_m = d.ray.min_depth_m = p.min_depth
+    d.pre.max_depth_m = d.integ.max_depth_m = p.max_depth
+    return d

@@ -1,0 +1,1025 @@
+// kf_oracle.cpp — CPU ORACLE.  TEST INFRASTRUCTURE ONLY.
+//
+// A plain C++ (g++, OpenMP) restatement of the reference's per-frame hot path
+// under the configuration's parameterisation, used by tests/ as the parity
+// checker, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  The
+// product (libkinfu) never links or calls it.
+//
+// Each function cites the reference code it restates (paths relative to the
+// OpenFusion source tree).  The reference itself cannot be compiled here (CUDA,
+// cutil, CImg, GLUT and camera SDKs are absent), so nothing is built under
+// oracle/_ref; the restatement is pinned by closed forms, invariants and brute
+// force in tests/ instead.
+//
+// Two instantiations:
+//   * Real = float : evaluates the same FP32 expressions in the same order as the
+//     device kernels (compiled with -ffp-contract=off, IEEE division and sqrt), so
+//     masks, counts and most values agree bit for bit;
+//   * Real = double: the FP64 oracle (FP64 maps, FP64 volume, FP64 ICP) used to
+//     report the error of the FP32/FP16 pipelines.
+#include <math.h>
+#include <omp.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "../include/kinfu.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Scalar helpers.
+
+// e^-x: base-2 range reduction and a degree-8 Taylor polynomial of e^-y on
+// [0, ln 2) in float (the product's definition of the bilateral weight); the
+// double instantiation uses exp().
+inline float exp_neg_f(float x) {
+  if (!(x < 80.0f)) return 0.0f;
+  // n = floor(x / ln2); r = x - n ln2 by Cody-Waite (ln2_hi has 16 significant
+  // bits, so n * ln2_hi and x - n * ln2_hi are exact for n < 128).
+  const float n = floorf(x * 1.44269504f);
+  const float y = (x - n * 0.693145752f) - n * 1.42860677e-06f;
+  float p = 2.48015873e-05f;
+  p = p * y - 1.98412698e-04f;
+  p = p * y + 1.38888889e-03f;
+  p = p * y - 8.33333333e-03f;
+  p = p * y + 4.16666667e-02f;
+  p = p * y - 1.66666667e-01f;
+  p = p * y + 5.00000000e-01f;
+  p = p * y - 1.0f;
+  p = p * y + 1.0f;
+  return ldexpf(p, -(int)n);
+}
+inline float exp_neg(float x) { return exp_neg_f(x); }
+inline double exp_neg(double x) { return exp(-x); }
+
+// Round half up: cutil's fracf(a) < 0.5 ? floor : ceil (Source/include/CudaUtilities.cuh:61-64).
+template <typename R>
+inline R round_half_up(R a) {
+  const R fl = floor(a);
+  return (a - fl) < R(0.5) ? fl : ceil(a);
+}
+
+// IEEE binary16 round-to-nearest-even from float, and back (exact).
+inline uint16_t f2h(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t ex = (x >> 23) & 0xffu;
+  uint32_t mant = x & 0x7fffffu;
+  if (ex == 0xffu) return (uint16_t)(sign | 0x7c00u | (mant ? 0x200u | (mant >> 13) : 0u));
+  const int e = (int)ex - 127 + 15;
+  if (e >= 31) return (uint16_t)(sign | 0x7c00u);
+  if (e <= 0) {
+    if (e < -10) return (uint16_t)sign;
+    mant |= 0x800000u;
+    const int shift = 14 - e;
+    uint32_t hm = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u), halfway = 1u << (shift - 1);
+    if (rem > halfway || (rem == halfway && (hm & 1u))) ++hm;
+    return (uint16_t)(sign | hm);
+  }
+  uint32_t h = sign | ((uint32_t)e << 10) | (mant >> 13);
+  const uint32_t rem = mant & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;
+  return (uint16_t)h;
+}
+inline float h2f(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  const uint32_t ex = (h >> 10) & 0x1fu, mant = h & 0x3ffu;
+  uint32_t bits;
+  if (ex == 0) {
+    if (mant == 0) {
+      bits = sign;
+    } else {
+      const float v = (float)mant * 5.9604644775390625e-08f;  // 2^-24
+      return sign ? -v : v;
+    }
+  } else if (ex == 31) {
+    bits = sign | 0x7f800000u | (mant << 13);
+  } else {
+    bits = sign | ((ex - 15 + 127) << 23) | (mant << 13);
+  }
+  float f;
+  memcpy(&f, &bits, 4);
+  return f;
+}
+
+template <typename R>
+struct V3 {
+  R x, y, z;
+};
+template <typename R>
+inline V3<R> sub(V3<R> a, V3<R> b) {
+  return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <typename R>
+inline R dot(V3<R> a, V3<R> b) {
+  return (a.x * b.x + a.y * b.y) + a.z * b.z;
+}
+template <typename R>
+inline V3<R> cross(V3<R> a, V3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <typename R>
+inline bool finite3(V3<R> v) {
+  return v.x == v.x && v.y == v.y && v.z == v.z;
+}
+
+// Pose in the working precision: R (row-major) and t.
+template <typename R>
+struct Pose {
+  R r[9];
+  R t[3];
+};
+template <typename R>
+inline V3<R> rot(const Pose<R>& p, V3<R> v) {
+  return {(p.r[0] * v.x + p.r[1] * v.y) + p.r[2] * v.z, (p.r[3] * v.x + p.r[4] * v.y) + p.r[5] * v.z,
+          (p.r[6] * v.x + p.r[7] * v.y) + p.r[8] * v.z};
+}
+template <typename R>
+inline V3<R> xform(const Pose<R>& p, V3<R> v) {
+  V3<R> a = rot(p, v);
+  return {a.x + p.t[0], a.y + p.t[1], a.z + p.t[2]};
+}
+
+// Rigid inverse (R^T, -R^T t) and 4x4 product, FP64 (replaces the Eigen FP32
+// inverse of Source/src/General.cpp:27-36 / Source/src/ICP.cpp:106-115).
+void rigid_inverse(const double* T, double* Ti) {
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) Ti[r * 4 + c] = T[c * 4 + r];
+    Ti[r * 4 + 3] = -((T[0 * 4 + r] * T[3] + T[1 * 4 + r] * T[7]) + T[2 * 4 + r] * T[11]);
+  }
+  Ti[12] = Ti[13] = Ti[14] = 0.0;
+  Ti[15] = 1.0;
+}
+void mat4_mul(const double* A, const double* B, double* C) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      C[i * 4 + j] = ((A[i * 4 + 0] * B[0 * 4 + j] + A[i * 4 + 1] * B[1 * 4 + j]) + A[i * 4 + 2] * B[2 * 4 + j]) +
+                     A[i * 4 + 3] * B[3 * 4 + j];
+}
+
+// Working-precision poses from T_wc: FP32 roundings for float, exact for double.
+template <typename R>
+void poses_from(const double* Twc, Pose<R>& wc, Pose<R>& cw) {
+  double Ti[16];
+  rigid_inverse(Twc, Ti);
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) {
+      wc.r[r * 3 + c] = (R)Twc[r * 4 + c];
+      cw.r[r * 3 + c] = (R)Ti[r * 4 + c];
+    }
+    wc.t[r] = (R)Twc[r * 4 + 3];
+    cw.t[r] = (R)Ti[r * 4 + 3];
+  }
+}
+
+struct Intr {
+  double fx, fy, cx, cy;
+  int W, H;
+};
+template <typename R>
+struct IntrR {
+  R fx, fy, cx, cy;
+  int W, H;
+};
+
+// Pyramid intrinsics (float arithmetic, as the product computes them).
+kf_intrinsics level_intr(const kf_intrinsics& k0, int l) {
+  kf_intrinsics k = k0;
+  const float scale = 1.0f / (float)(1 << l);
+  k.fx = k0.fx * scale;
+  k.fy = k0.fy * scale;
+  k.cx = (k0.cx + 0.5f) * scale - 0.5f;
+  k.cy = (k0.cy + 0.5f) * scale - 0.5f;
+  k.width = k0.width >> l;
+  k.height = k0.height >> l;
+  return k;
+}
+template <typename R>
+IntrR<R> intr_r(const kf_intrinsics& k) {
+  return {(R)k.fx, (R)k.fy, (R)k.cx, (R)k.cy, k.width, k.height};
+}
+
+// ---------------------------------------------------------------------------
+// Maps in the working precision (planar, NaN = invalid, depth 0 = invalid).
+template <typename R>
+struct Level {
+  int W = 0, H = 0;
+  std::vector<R> depth, vx, vy, vz, nx, ny, nz;
+  void alloc(int w, int h, bool with_depth) {
+    W = w;
+    H = h;
+    const size_t P = (size_t)w * h;
+    const R nan = (R)NAN;
+    if (with_depth) depth.assign(P, R(0));
+    vx.assign(P, nan); vy.assign(P, nan); vz.assign(P, nan);
+    nx.assign(P, nan); ny.assign(P, nan); nz.assign(P, nan);
+  }
+};
+template <typename R>
+struct Maps {
+  int levels = 0;
+  std::vector<R> raylen;
+  Level<R> lv[KF_MAX_LEVELS];
+};
+
+// ---------------------------------------------------------------------------
+// Preprocess.  Depth conversion and validity follow d_convertDepthToWorldVertices
+// (Source/src/CameraSupport/CameraHandler.cu:65-89, range test :80-81); the
+// bilateral filter (Tomasi & Manduchi 1998 / Newcombe et al. 2011 eq. 2) and
+// block-average pyramid are the configuration's additions; normals follow
+// d_convertDepthToWorldNormals (CameraHandler.cu:91-132) with the cross product
+// ordered so that normals face the camera.
+template <typename R>
+void preprocess(const kf_intrinsics& k0, const kf_preprocess_params& p, const uint16_t* raw, Maps<R>& m,
+                int levels) {
+  const int W = k0.width, H = k0.height;
+  m.levels = levels;
+  m.raylen.assign((size_t)W * H, R(0));
+  for (int l = 0; l < levels; ++l) {
+    kf_intrinsics kl = level_intr(k0, l);
+    m.lv[l].alloc(kl.width, kl.height, true);
+  }
+  std::vector<R> d0((size_t)W * H);
+  const R scale = (R)p.depth_scale, dmin = (R)p.min_depth_m, dmax = (R)p.max_depth_m;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < W * H; ++i) {
+    R d = (R)raw[i] * scale;
+    if (!(d >= dmin && d <= dmax)) d = R(0);
+    d0[i] = d;
+  }
+  const int Rr = p.bilateral_radius;
+  R inv2ss, inv2sr;
+  if (sizeof(R) == 4) {
+    inv2ss = (R)(0.5f / (p.sigma_space_px * p.sigma_space_px));
+    inv2sr = (R)(0.5f / (p.sigma_range_m * p.sigma_range_m));
+  } else {
+    inv2ss = R(0.5) / ((R)p.sigma_space_px * (R)p.sigma_space_px);
+    inv2sr = R(0.5) / ((R)p.sigma_range_m * (R)p.sigma_range_m);
+  }
+  const IntrR<R> K = intr_r<R>(k0);
+  Level<R>& L0 = m.lv[0];
+#pragma omp parallel for schedule(static)
+  for (int y = 0; y < H; ++y) {
+    for (int x = 0; x < W; ++x) {
+      const int idx = y * W + x;
+      const R dc = d0[idx];
+      if (dc == R(0)) {
+        m.raylen[idx] = R(0);
+        L0.depth[idx] = R(0);
+        continue;
+      }
+      const R xn = ((R)x - K.cx) / K.fx;
+      const R yn = ((R)y - K.cy) / K.fy;
+      const R lam = sqrt((xn * xn + yn * yn) + R(1));
+      m.raylen[idx] = dc * lam;
+      R sw = R(0), swd = R(0);
+      for (int dy = -Rr; dy <= Rr; ++dy) {
+        const int qy = y + dy;
+        if (qy < 0 || qy >= H) continue;
+        for (int dx = -Rr; dx <= Rr; ++dx) {
+          const int qx = x + dx;
+          if (qx < 0 || qx >= W) continue;
+          const R dq = d0[qy * W + qx];
+          if (dq == R(0)) continue;
+          const R diff = dq - dc;
+          const R arg = (R)(dx * dx + dy * dy) * inv2ss + (diff * diff) * inv2sr;
+          const R w = exp_neg(arg);
+          sw = sw + w;
+          swd = swd + w * dq;
+        }
+      }
+      L0.depth[idx] = swd / sw;
+    }
+  }
+  const R gate = (R)p.pyramid_gate_m;
+  for (int l = 1; l < levels; ++l) {
+    const Level<R>& fi = m.lv[l - 1];
+    Level<R>& co = m.lv[l];
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < co.H; ++y) {
+      for (int x = 0; x < co.W; ++x) {
+        const R c = fi.depth[(2 * y) * fi.W + 2 * x];
+        R r = R(0);
+        if (c != R(0)) {
+          R s = R(0), n = R(0);
+          for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < 2; ++i) {
+              const int sx = 2 * x + i, sy = 2 * y + j;
+              if (sx >= fi.W || sy >= fi.H) continue;
+              const R d = fi.depth[sy * fi.W + sx];
+              if (d > R(0) && fabs(d - c) < gate) {
+                s = s + d;
+                n = n + R(1);
+              }
+            }
+          r = s / n;
+        }
+        co.depth[y * co.W + x] = r;
+      }
+    }
+  }
+  for (int l = 0; l < levels; ++l) {
+    const IntrR<R> Kl = intr_r<R>(level_intr(k0, l));
+    Level<R>& lv = m.lv[l];
+    auto bp = [&](int x, int y) -> V3<R> {
+      const R d = lv.depth[y * lv.W + x];
+      if (d == R(0)) return {(R)NAN, (R)NAN, (R)NAN};
+      const R xn = ((R)x - Kl.cx) / Kl.fx;
+      const R yn = ((R)y - Kl.cy) / Kl.fy;
+      return {xn * d, yn * d, d};
+    };
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < lv.H; ++y) {
+      for (int x = 0; x < lv.W; ++x) {
+        const int idx = y * lv.W + x;
+        const V3<R> v = bp(x, y);
+        lv.vx[idx] = v.x;
+        lv.vy[idx] = v.y;
+        lv.vz[idx] = v.z;
+        V3<R> n = {(R)NAN, (R)NAN, (R)NAN};
+        if (v.z == v.z && x + 1 < lv.W && y + 1 < lv.H) {
+          const V3<R> vr = bp(x + 1, y), vd = bp(x, y + 1);
+          if (vr.z == vr.z && vd.z == vd.z) {
+            const V3<R> c = cross(sub(vd, v), sub(vr, v));
+            const R len = sqrt(dot(c, c));
+            if (len > R(0)) {
+              const R inv = R(1) / len;
+              n = {c.x * inv, c.y * inv, c.z * inv};
+            }
+          }
+        }
+        lv.nx[idx] = n.x;
+        lv.ny[idx] = n.y;
+        lv.nz[idx] = n.z;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Volume.  FP32: {float f, float w}; FP16: {half f | u16 w << 16}; FP64: {double f, double w}.
+struct VolDesc {
+  int n;
+  double extent, ox, oy, oz, trunc, maxw;
+  int format;  // KF_VOXEL_F32 / KF_VOXEL_F16 / 2 = FP64
+};
+constexpr int kFmtF64 = 2;
+
+size_t voxel_bytes(int fmt) { return fmt == KF_VOXEL_F16 ? 4 : (fmt == KF_VOXEL_F32 ? 8 : 16); }
+
+void volume_reset(const VolDesc& d, void* vox) {
+  const size_t N = (size_t)d.n * d.n * d.n;
+  if (d.format == KF_VOXEL_F16) {
+    uint32_t* p = (uint32_t*)vox;
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < N; ++i) p[i] = 0x3C00u;
+  } else if (d.format == KF_VOXEL_F32) {
+    float* p = (float*)vox;
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < N; ++i) {
+      p[2 * i] = 1.0f;
+      p[2 * i + 1] = 0.0f;
+    }
+  } else {
+    double* p = (double*)vox;
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < N; ++i) {
+      p[2 * i] = 1.0;
+      p[2 * i + 1] = 0.0;
+    }
+  }
+}
+
+// Projective TSDF sweep over every voxel (no frustum culling), restating
+// d_integrateVolume (Source/src/VolumeIntegrator.cu:86-187): voxel centre ->
+// camera -> nearest pixel (round half up), 2-pixel frustum margin and depth
+// range (:12-16, :131), invalid pixel -> skip (:138-140), SDF = |v - c| - |p - c|
+// (:146) with |v - c| the pixel's ray length, skip if SDF < -T, else
+// tsdf = min(SDF, T) (:148-160), running average with unit sample weight and
+// capped weight (:165-166); stored normalised by T.  Parallel over (x, y) columns.
+template <typename R>
+void integrate(const VolDesc& d, void* vox, const kf_intrinsics& k0, const kf_integrate_params& ip,
+               const R* raylen, const double* Twc, uint64_t* updated, uint32_t* mask) {
+  Pose<R> wc, cw;
+  poses_from<R>(Twc, wc, cw);
+  const int n = d.n;
+  R h, trunc, inv_trunc, maxw;
+  if (sizeof(R) == 4) {
+    h = (R)((float)d.extent / (float)n);
+    trunc = (R)(float)d.trunc;
+    inv_trunc = (R)(1.0f / (float)d.trunc);
+    maxw = (R)(float)d.maxw;
+  } else {
+    h = (R)d.extent / (R)n;
+    trunc = (R)d.trunc;
+    inv_trunc = R(1) / trunc;
+    maxw = (R)d.maxw;
+  }
+  const R ox = (R)d.ox, oy = (R)d.oy, oz = (R)d.oz;
+  const IntrR<R> K = intr_r<R>(k0);
+  const R dmin = (R)ip.min_depth_m, dmax = (R)ip.max_depth_m;
+  const int W = k0.width, H = k0.height;
+  uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+  for (int c = 0; c < n * n; ++c) {
+    const int ix = c / n, iy = c - ix * n;
+    const R wx = ox + ((R)ix + R(0.5)) * h;
+    const R wy = oy + ((R)iy + R(0.5)) * h;
+    const R ax = (cw.r[0] * wx + cw.r[1] * wy) + cw.t[0];
+    const R ay = (cw.r[3] * wx + cw.r[4] * wy) + cw.t[1];
+    const R az = (cw.r[6] * wx + cw.r[7] * wy) + cw.t[2];
+    for (int k = 0; k < n; ++k) {
+      const R wz = oz + ((R)k + R(0.5)) * h;
+      const R px = ax + cw.r[2] * wz;
+      const R py = ay + cw.r[5] * wz;
+      const R pz = az + cw.r[8] * wz;
+      if (!(pz >= dmin && pz <= dmax)) continue;
+      const R iz = R(1) / pz;
+      const R u = (K.fx * px + K.cx * pz) * iz;
+      const R v = (K.fy * py + K.cy * pz) * iz;
+      const R ru = round_half_up(u), rv = round_half_up(v);
+      if (!(ru >= R(2) && ru < (R)(W - 2) && rv >= R(2) && rv < (R)(H - 2))) continue;
+      const R rl = raylen[(int)rv * W + (int)ru];
+      if (!(rl > R(0))) continue;
+      const R dist = sqrt((px * px + py * py) + pz * pz);
+      const R sdf = rl - dist;
+      if (sdf < -trunc) continue;
+      const R f = std::max(std::min(sdf * inv_trunc, R(1)), R(-1));
+      const size_t idx = (size_t)c * n + k;
+      if (d.format == KF_VOXEL_F16) {
+        uint32_t& w = ((uint32_t*)vox)[idx];
+        const float fo = h2f((uint16_t)(w & 0xffffu));
+        const float wo = (float)(w >> 16);
+        const float fn = (fo * wo + (float)f) / (wo + 1.0f);
+        const float wn = std::min(wo + 1.0f, (float)maxw);
+        w = (uint32_t)f2h(fn) | ((uint32_t)wn << 16);
+      } else if (d.format == KF_VOXEL_F32) {
+        float* p = (float*)vox + 2 * idx;
+        const float fo = p[0], wo = p[1];
+        p[0] = (fo * wo + (float)f) / (wo + 1.0f);
+        p[1] = std::min(wo + 1.0f, (float)maxw);
+      } else {
+        double* p = (double*)vox + 2 * idx;
+        const double fo = p[0], wo = p[1];
+        p[0] = (fo * wo + (double)f) / (wo + 1.0);
+        p[1] = std::min(wo + 1.0, (double)maxw);
+      }
+      ++total;
+      if (mask) {
+#pragma omp atomic
+        mask[idx >> 5] |= 1u << (idx & 31);
+      }
+    }
+  }
+  if (updated) *updated += total;
+}
+
+// ---------------------------------------------------------------------------
+// Raycast, restating d_raycast (Source/src/Raycaster.cu:97-230): eye ray through
+// the pixel, slab test against the volume box (:35-55, :129-138), march until the
+// first tsdf <= 0 (:152-188) and a gradient normal (:64-94).  As configured, the
+// samples are trilinear, the zero crossing is refined linearly (the intent of the
+// disabled getSurfaceVertex, :334-350), the march takes step_far*T while the
+// sample is +1 and step_near voxels otherwise, and normals are trilinear central
+// differences.
+template <typename R>
+struct VolView {
+  const VolDesc* d;
+  const void* vox;
+  R h, inv_h, ox, oy, oz;
+  R f(size_t i) const {
+    if (d->format == KF_VOXEL_F16) return (R)h2f((uint16_t)(((const uint32_t*)vox)[i] & 0xffffu));
+    if (d->format == KF_VOXEL_F32) return (R)((const float*)vox)[2 * i];
+    return (R)((const double*)vox)[2 * i];
+  }
+  bool sample(V3<R> p, R& out) const {
+    const R gx = (p.x - ox) * inv_h - R(0.5);
+    const R gy = (p.y - oy) * inv_h - R(0.5);
+    const R gz = (p.z - oz) * inv_h - R(0.5);
+    const R lim = (R)(d->n - 1);
+    if (!(gx >= R(0) && gx < lim && gy >= R(0) && gy < lim && gz >= R(0) && gz < lim)) return false;
+    const int ix = (int)gx, iy = (int)gy, iz = (int)gz;
+    const R tx = gx - (R)ix, ty = gy - (R)iy, tz = gz - (R)iz;
+    const size_t n = (size_t)d->n;
+    const size_t b = ((size_t)ix * n + (size_t)iy) * n + (size_t)iz, sy = n, sx = n * n;
+    auto lerp = [](R a, R c, R t) { return a + (c - a) * t; };
+    const R c00 = lerp(f(b), f(b + 1), tz), c01 = lerp(f(b + sy), f(b + sy + 1), tz);
+    const R c10 = lerp(f(b + sx), f(b + sx + 1), tz), c11 = lerp(f(b + sx + sy), f(b + sx + sy + 1), tz);
+    const R c0 = lerp(c00, c01, ty), c1 = lerp(c10, c11, ty);
+    out = lerp(c0, c1, tx);
+    return true;
+  }
+};
+
+template <typename R>
+void raycast(const VolDesc& d, const void* vox, const kf_intrinsics& k0, const kf_raycast_params& rp,
+             const double* Twc, Maps<R>& model, int levels, uint64_t* steps_out) {
+  Pose<R> wc, cw;
+  poses_from<R>(Twc, wc, cw);
+  VolView<R> V;
+  V.d = &d;
+  V.vox = vox;
+  R step_far, step_near;
+  if (sizeof(R) == 4) {
+    const float hf = (float)d.extent / (float)d.n;
+    V.h = (R)hf;
+    V.inv_h = (R)((float)d.n / (float)d.extent);
+    step_far = (R)(rp.step_far * (float)d.trunc);
+    step_near = (R)(rp.step_near * hf);
+  } else {
+    V.h = (R)d.extent / (R)d.n;
+    V.inv_h = (R)d.n / (R)d.extent;
+    step_far = (R)rp.step_far * (R)d.trunc;
+    step_near = (R)rp.step_near * V.h;
+  }
+  V.ox = (R)d.ox;
+  V.oy = (R)d.oy;
+  V.oz = (R)d.oz;
+  const IntrR<R> K = intr_r<R>(k0);
+  const int W = k0.width, H = k0.height;
+  model.levels = levels;
+  for (int l = 0; l < levels; ++l) {
+    kf_intrinsics kl = level_intr(k0, l);
+    model.lv[l].alloc(kl.width, kl.height, false);
+  }
+  Level<R>& out = model.lv[0];
+  const R nanv = (R)NAN;
+  // Voxel-centre box shrunk by 0.01 voxel (first/last samples strictly inside).
+  const R blo_ = (R)0.51f * V.h, bhi_ = ((R)d.n - (R)0.51f) * V.h;
+  const R blo[3] = {V.ox + blo_, V.oy + blo_, V.oz + blo_};
+  const R bhi[3] = {V.ox + bhi_, V.oy + bhi_, V.oz + bhi_};
+  uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : total)
+  for (int y = 0; y < H; ++y) {
+    for (int x = 0; x < W; ++x) {
+      const int idx = y * W + x;
+      const R xn = ((R)x - K.cx) / K.fx;
+      const R yn = ((R)y - K.cy) / K.fy;
+      const V3<R> dw = rot(wc, V3<R>{xn, yn, R(1)});
+      const R len = sqrt(dot(dw, dw));
+      const R il = R(1) / len;
+      const V3<R> dir = {dw.x * il, dw.y * il, dw.z * il};
+      const V3<R> o = {wc.t[0], wc.t[1], wc.t[2]};
+      const R oo[3] = {o.x, o.y, o.z}, dd[3] = {dir.x, dir.y, dir.z};
+      R tnear = (R)rp.min_depth_m * len, tfar = (R)3.0e38f;
+      bool hit = true;
+      for (int k = 0; k < 3; ++k) {
+        if (dd[k] != R(0)) {
+          const R t1 = (blo[k] - oo[k]) / dd[k];
+          const R t2 = (bhi[k] - oo[k]) / dd[k];
+          tnear = std::max(tnear, std::min(t1, t2));
+          tfar = std::min(tfar, std::max(t1, t2));
+        } else if (!(oo[k] >= blo[k] && oo[k] <= bhi[k])) {
+          hit = false;
+        }
+      }
+      V3<R> vert = {nanv, nanv, nanv}, nrm = vert;
+      if (hit && tnear < tfar) {
+        R t = tnear, fp;
+        V3<R> p = {o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z};
+        bool ok = V.sample(p, fp) && fp >= R(0);
+        for (int s = 0; ok && s < rp.max_steps; ++s) {
+          ++total;
+          const R st = (fp >= R(1)) ? step_far : step_near;
+          const R tn = t + st;
+          if (tn > tfar) break;
+          p = {o.x + tn * dir.x, o.y + tn * dir.y, o.z + tn * dir.z};
+          R fc;
+          if (!V.sample(p, fc)) break;
+          if (fc < R(0)) {
+            const R ts = t + st * (fp / (fp - fc));
+            const V3<R> ph = {o.x + ts * dir.x, o.y + ts * dir.y, o.z + ts * dir.z};
+            R gxp, gxm, gyp, gym, gzp, gzm;
+            const bool g = V.sample({ph.x + V.h, ph.y, ph.z}, gxp) && V.sample({ph.x - V.h, ph.y, ph.z}, gxm) &&
+                           V.sample({ph.x, ph.y + V.h, ph.z}, gyp) && V.sample({ph.x, ph.y - V.h, ph.z}, gym) &&
+                           V.sample({ph.x, ph.y, ph.z + V.h}, gzp) && V.sample({ph.x, ph.y, ph.z - V.h}, gzm);
+            if (g) {
+              const V3<R> gr = {gxp - gxm, gyp - gym, gzp - gzm};
+              const R gl = sqrt(dot(gr, gr));
+              if (gl > R(0)) {
+                const R ig = R(1) / gl;
+                nrm = {gr.x * ig, gr.y * ig, gr.z * ig};
+                vert = ph;
+              }
+            }
+            break;
+          }
+          t = tn;
+          fp = fc;
+        }
+      }
+      out.vx[idx] = vert.x;
+      out.vy[idx] = vert.y;
+      out.vz[idx] = vert.z;
+      out.nx[idx] = nrm.x;
+      out.ny[idx] = nrm.y;
+      out.nz[idx] = nrm.z;
+    }
+  }
+  for (int l = 1; l < levels; ++l) {
+    Level<R>& o = model.lv[l];
+    for (int y = 0; y < o.H; ++y)
+      for (int x = 0; x < o.W; ++x) {
+        const int i = y * o.W + x, j = (y << l) * W + (x << l);
+        o.vx[i] = out.vx[j];
+        o.vy[i] = out.vy[j];
+        o.vz[i] = out.vz[j];
+        o.nx[i] = out.nx[j];
+        o.ny[i] = out.ny[j];
+        o.nz[i] = out.nz[j];
+      }
+  }
+  if (steps_out) *steps_out += total;
+}
+
+// ---------------------------------------------------------------------------
+// ICP.  Association restates d_find_corresponding (Source/src/ICP.cu:116-205):
+// valid live depth at the model pixel (:145-149), model vertex to the live camera
+// with the inverse estimate (:152), projection and round half up (:155-162),
+// frustum (:90-95), gather of live vertex/normal (:172-177), distance and normal
+// gates (:179-181), row A = [m x d, m], b = m.(s - d) (:186-201).  The sums are
+// brute-force FP64 over pixels (the reference sums FP32 block partials on the
+// host, Source/src/ICP.cpp:128-140); the solve is FP64 Cholesky with a relative
+// pivot guard (replacing the FP32 determinant guard and LLT, ICP.cpp:152-160),
+// followed by the sign flip of the rotation (:162-168) and the Euler-ZYX
+// increment left-multiplied onto the pose (Source/src/ICP.cu:266-305).
+template <typename R>
+int associate(const Level<R>& L, const Level<R>& M, const IntrR<R>& K, const Pose<R>& wc, const Pose<R>& cw,
+              R dist_thr, R cos_thr, int i, R* A, R& b) {
+  const R lz = L.vz[i];
+  const V3<R> s = {M.vx[i], M.vy[i], M.vz[i]};
+  const V3<R> m = {M.nx[i], M.ny[i], M.nz[i]};
+  if (!(lz == lz) || !finite3(s) || !finite3(m)) return 0;
+  const V3<R> sc = xform(cw, s);
+  if (!(sc.z > (R)1e-6f)) return 1;
+  const R iz = R(1) / sc.z;
+  const R u = (K.fx * sc.x + K.cx * sc.z) * iz;
+  const R v = (K.fy * sc.y + K.cy * sc.z) * iz;
+  const R ru = round_half_up(u), rv = round_half_up(v);
+  if (!(ru >= R(0) && ru < (R)K.W && rv >= R(0) && rv < (R)K.H)) return 1;
+  const int j = (int)rv * K.W + (int)ru;
+  const V3<R> dc = {L.vx[j], L.vy[j], L.vz[j]};
+  const V3<R> nc = {L.nx[j], L.ny[j], L.nz[j]};
+  if (!finite3(dc) || !finite3(nc)) return 1;
+  const V3<R> d = xform(wc, dc);
+  const V3<R> n = rot(wc, nc);
+  const V3<R> df = sub(d, s);
+  if (!(sqrt(dot(df, df)) < dist_thr)) return 1;
+  const R cs = dot(n, m);
+  if (!(cs > cos_thr && cs < R(1) + (R)1e-6f)) return 1;
+  A[0] = m.y * d.z - m.z * d.y;
+  A[1] = m.z * d.x - m.x * d.z;
+  A[2] = m.x * d.y - m.y * d.x;
+  A[3] = m.x;
+  A[4] = m.y;
+  A[5] = m.z;
+  b = (m.x * (s.x - d.x) + m.y * (s.y - d.y)) + m.z * (s.z - d.z);
+  return 2;
+}
+
+template <typename R>
+void icp_reduce(const kf_intrinsics& k0, int level, float dist_thr_m, float normal_deg, const Maps<R>& live,
+                const Maps<R>& model, const double* Twc, double* sums, uint8_t* mask) {
+  Pose<R> wc, cw;
+  poses_from<R>(Twc, wc, cw);
+  const Level<R>& L = live.lv[level];
+  const Level<R>& M = model.lv[level];
+  const IntrR<R> K = intr_r<R>(level_intr(k0, level));
+  const R dist_thr = (R)dist_thr_m;
+  const R cos_thr = sizeof(R) == 4 ? (R)(float)cos((double)normal_deg * kPi / 180.0)
+                                   : (R)cos((double)normal_deg * kPi / 180.0);
+  const int P = L.W * L.H;
+  const int nt = omp_get_max_threads();
+  std::vector<double> part((size_t)nt * 30, 0.0);
+#pragma omp parallel
+  {
+    double* acc = &part[(size_t)omp_get_thread_num() * 30];
+#pragma omp for schedule(static)
+    for (int i = 0; i < P; ++i) {
+      R A[6], b;
+      const int r = associate(L, M, K, wc, cw, dist_thr, cos_thr, i, A, b);
+      if (mask) mask[i] = (uint8_t)(r == 2);
+      if (r == 0) continue;
+      acc[29] += 1.0;
+      if (r != 2) continue;
+      acc[28] += 1.0;
+      int k = 0;
+      for (int p = 0; p < 6; ++p)
+        for (int q = p; q < 6; ++q) acc[k++] += (double)(A[p] * A[q]);
+      for (int p = 0; p < 6; ++p) acc[21 + p] += (double)(A[p] * b);
+      acc[27] += (double)(b * b);
+    }
+  }
+  for (int q = 0; q < 30; ++q) {
+    double s = 0.0;
+    for (int t = 0; t < nt; ++t) s += part[(size_t)t * 30 + q];
+    sums[q] = s;
+  }
+}
+
+bool cholesky_solve6(const double* u21, const double* g, double rel_eps, double* x) {
+  double A[6][6], L[6][6];
+  int k = 0;
+  double dmax = 0.0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j) {
+      A[i][j] = A[j][i] = u21[k++];
+    }
+  for (int i = 0; i < 6; ++i) dmax = std::max(dmax, A[i][i]);
+  for (int j = 0; j < 6; ++j) {
+    double d = A[j][j];
+    for (int m = 0; m < j; ++m) d -= L[j][m] * L[j][m];
+    if (!(d > rel_eps * dmax)) return false;
+    L[j][j] = sqrt(d);
+    for (int i = j + 1; i < 6; ++i) {
+      double s = A[i][j];
+      for (int m = 0; m < j; ++m) s -= L[i][m] * L[j][m];
+      L[i][j] = s / L[j][j];
+    }
+  }
+  double y[6];
+  for (int i = 0; i < 6; ++i) {
+    double s = g[i];
+    for (int m = 0; m < i; ++m) s -= L[i][m] * y[m];
+    y[i] = s / L[i][i];
+  }
+  for (int i = 5; i >= 0; --i) {
+    double s = y[i];
+    for (int m = i + 1; m < 6; ++m) s -= L[m][i] * x[m];
+    x[i] = s / L[i][i];
+  }
+  return true;
+}
+
+void euler_zyx(double a, double b, double g, double tx, double ty, double tz, double* D) {
+  const double ca = cos(a), sa = sin(a), cb = cos(b), sb = sin(b), cg = cos(g), sg = sin(g);
+  const double M[16] = {cg * cb, -sg * ca + (cg * sb) * sa, sg * sa + (cg * sb) * ca, tx,
+                        sg * cb, cg * ca + (sg * sb) * sa, -cg * sa + (sg * sb) * ca, ty,
+                        -sb,     cb * sa,                  cb * ca,                  tz,
+                        0.0,     0.0,                      0.0,                      1.0};
+  memcpy(D, M, sizeof(M));
+}
+
+// One solve + compose step; returns false (pose kept) if singular.
+bool icp_step(const double* sums, double* T) {
+  double x[6];
+  if (!(sums[28] >= 6.0) || !cholesky_solve6(sums, sums + 21, 1e-9, x)) return false;
+  double D[16], Tn[16];
+  euler_zyx(-x[0], -x[1], -x[2], x[3], x[4], x[5], D);
+  mat4_mul(D, T, Tn);
+  memcpy(T, Tn, sizeof(Tn));
+  T[12] = T[13] = T[14] = 0.0;
+  T[15] = 1.0;
+  return true;
+}
+
+template <typename R>
+void icp_track(const kf_intrinsics& k0, const kf_icp_params& p, const Maps<R>& live, const Maps<R>& model,
+               const double* guess, double* out, kf_icp_stats* st) {
+  double T[16];
+  memcpy(T, guess, sizeof(T));
+  kf_icp_stats s;
+  memset(&s, 0, sizeof(s));
+  bool stop = false;
+  for (int l = p.levels - 1; l >= 0 && !stop; --l) {
+    for (int it = 0; it < p.iterations[l] && !stop; ++it) {
+      double sums[30];
+      icp_reduce<R>(k0, l, p.dist_thresh_m, p.normal_thresh_deg, live, model, T, sums, nullptr);
+      s.inliers = (int)sums[28];
+      s.candidates = (int)sums[29];
+      s.residual_rms = sums[28] > 0 ? sqrt(sums[27] / sums[28]) : 0.0;
+      s.level_residual[l] = s.residual_rms;
+      s.level_inliers[l] = s.inliers;
+      if (icp_step(sums, T)) {
+        ++s.iterations;
+      } else {
+        s.status = 1;
+        stop = true;
+      }
+    }
+  }
+  memcpy(out, T, sizeof(T));
+  if (st) *st = s;
+}
+
+// ---------------------------------------------------------------------------
+// Frame pipeline (RunApp::display order rotated to preprocess -> track ->
+// integrate -> raycast; Source/src/MainApp.cpp:774-970) with the
+// constant-velocity guess T1 (T0^-1 T1).
+struct PipelineBase {
+  virtual ~PipelineBase() {}
+  kf_pipeline_desc d;
+  VolDesc vd;
+  std::vector<uint8_t> vox;
+  double Tcur[16], Tprev1[16], Tprev2[16];
+  int frame = 0;
+  kf_icp_stats stats;
+  double stage_ms[4];
+  virtual void process(const uint16_t* depth) = 0;
+  virtual void maps(kf_frame_maps* live, kf_frame_maps* model, int which_precision_float) = 0;
+};
+
+template <typename R>
+struct Pipeline : PipelineBase {
+  Maps<R> live, model;
+  void process(const uint16_t* depth) override {
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    const int levels = d.icp.levels;
+    auto t0 = now();
+    preprocess<R>(d.intr, d.pre, depth, live, levels);
+    auto t1 = now();
+    if (frame > 0) {
+      double Ti[16], Dl[16], G[16];
+      rigid_inverse(Tprev2, Ti);
+      mat4_mul(Ti, Tprev1, Dl);
+      mat4_mul(Tprev1, Dl, G);
+      icp_track<R>(d.intr, d.icp, live, model, G, Tcur, &stats);
+    }
+    auto t2 = now();
+    integrate<R>(vd, vox.data(), d.intr, d.integ, live.raylen.data(), Tcur, nullptr, nullptr);
+    auto t3 = now();
+    raycast<R>(vd, vox.data(), d.intr, d.ray, Tcur, model, levels, nullptr);
+    auto t4 = now();
+    memcpy(Tprev2, Tprev1, sizeof(Tcur));
+    memcpy(Tprev1, Tcur, sizeof(Tcur));
+    stage_ms[0] = ms(t0, t1);
+    stage_ms[1] = ms(t1, t2);
+    stage_ms[2] = ms(t2, t3);
+    stage_ms[3] = ms(t3, t4);
+    ++frame;
+  }
+  void maps(kf_frame_maps*, kf_frame_maps*, int) override {}
+};
+
+VolDesc to_vd(const kf_volume_desc& v, int fmt) {
+  VolDesc d;
+  d.n = v.n;
+  d.extent = v.extent_m;
+  d.ox = v.origin_m[0];
+  d.oy = v.origin_m[1];
+  d.oz = v.origin_m[2];
+  d.trunc = v.trunc_m;
+  d.maxw = v.max_weight;
+  d.format = fmt;
+  return d;
+}
+
+// Copies working-precision maps to caller host buffers laid out like kf_frame_maps.
+template <typename R>
+void export_maps(const Maps<R>& m, kf_frame_maps* out) {
+  if (!out) return;
+  auto cp = [](const std::vector<R>& src, float* dst) {
+    if (!dst || src.empty()) return;
+    for (size_t i = 0; i < src.size(); ++i) dst[i] = (float)src[i];
+  };
+  cp(m.raylen, out->raylen);
+  for (int l = 0; l < m.levels && l < out->levels; ++l) {
+    const Level<R>& s = m.lv[l];
+    kf_level_maps& o = out->level[l];
+    cp(s.depth, o.depth);
+    cp(s.vx, o.vx); cp(s.vy, o.vy); cp(s.vz, o.vz);
+    cp(s.nx, o.nx); cp(s.ny, o.ny); cp(s.nz, o.nz);
+  }
+}
+template <typename R>
+void import_maps(const kf_frame_maps* in, Maps<R>& m, bool live) {
+  m.levels = in->levels;
+  auto cp = [](const float* src, std::vector<R>& dst, size_t n) {
+    dst.resize(n);
+    for (size_t i = 0; i < n; ++i) dst[i] = (R)src[i];
+  };
+  if (live && in->raylen) cp(in->raylen, m.raylen, (size_t)in->level[0].width * in->level[0].height);
+  for (int l = 0; l < in->levels; ++l) {
+    const kf_level_maps& s = in->level[l];
+    Level<R>& o = m.lv[l];
+    o.W = s.width;
+    o.H = s.height;
+    const size_t P = (size_t)s.width * s.height;
+    if (s.depth) cp(s.depth, o.depth, P);
+    cp(s.vx, o.vx, P); cp(s.vy, o.vy, P); cp(s.vz, o.vz, P);
+    cp(s.nx, o.nx, P); cp(s.ny, o.ny, P); cp(s.nz, o.nz, P);
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C entry points (host pointers throughout).
+extern "C" {
+
+float kfo_exp_neg(float x) { return exp_neg_f(x); }
+uint16_t kfo_float_to_half(float f) { return f2h(f); }
+float kfo_half_to_float(uint16_t h) { return h2f(h); }
+kf_intrinsics kfo_intrinsics_level(const kf_intrinsics* k0, int32_t l) { return level_intr(*k0, l); }
+int32_t kfo_num_threads(void) { return omp_get_max_threads(); }
+
+// precision: 0 = FP32 restatement, 1 = FP64 oracle (outputs rounded to float).
+void kfo_preprocess(const kf_intrinsics* k0, const kf_preprocess_params* p, const uint16_t* depth, int32_t levels,
+                    int32_t precision, kf_frame_maps* out) {
+  if (precision == 0) {
+    Maps<float> m;
+    preprocess<float>(*k0, *p, depth, m, levels);
+    export_maps(m, out);
+  } else {
+    Maps<double> m;
+    preprocess<double>(*k0, *p, depth, m, levels);
+    export_maps(m, out);
+  }
+}
+
+void kfo_volume_reset(const kf_volume_desc* d, int32_t fmt, void* vox) { volume_reset(to_vd(*d, fmt), vox); }
+
+// fmt: KF_VOXEL_F32 / KF_VOXEL_F16 (FP32 arithmetic) or 2 (FP64 volume, FP64 arithmetic
+// on a double raylen given as float).
+void kfo_integrate(const kf_volume_desc* d, int32_t fmt, void* vox, const kf_intrinsics* k0,
+                   const kf_integrate_params* p, const float* raylen, const double* T_wc, uint64_t* updated,
+                   uint32_t* mask) {
+  const VolDesc vd = to_vd(*d, fmt);
+  if (fmt == kFmtF64) {
+    std::vector<double> rl((size_t)k0->width * k0->height);
+    for (size_t i = 0; i < rl.size(); ++i) rl[i] = raylen[i];
+    integrate<double>(vd, vox, *k0, *p, rl.data(), T_wc, updated, mask);
+  } else {
+    integrate<float>(vd, vox, *k0, *p, raylen, T_wc, updated, mask);
+  }
+}
+
+void kfo_raycast(const kf_volume_desc* d, int32_t fmt, const void* vox, const kf_intrinsics* k0,
+                 const kf_raycast_params* p, const double* T_wc, int32_t levels, kf_frame_maps* out,
+                 uint64_t* steps) {
+  const VolDesc vd = to_vd(*d, fmt);
+  if (fmt == kFmtF64) {
+    Maps<double> m;
+    raycast<double>(vd, vox, *k0, *p, T_wc, m, levels, steps);
+    export_maps(m, out);
+  } else {
+    Maps<float> m;
+    raycast<float>(vd, vox, *k0, *p, T_wc, m, levels, steps);
+    export_maps(m, out);
+  }
+}
+
+void kfo_icp_reduce(const kf_intrinsics* k0, int32_t level, const kf_icp_params* p, const kf_frame_maps* live,
+                    const kf_frame_maps* model, const double* T_wc, double* sums, uint8_t* mask) {
+  Maps<float> L, M;
+  import_maps(live, L, true);
+  import_maps(model, M, false);
+  icp_reduce<float>(*k0, level, p->dist_thresh_m, p->normal_thresh_deg, L, M, T_wc, sums, mask);
+}
+
+int32_t kfo_icp_solve_step(const double* sums, double* T_wc) { return icp_step(sums, T_wc) ? 1 : 0; }
+
+void kfo_icp_track(const kf_intrinsics* k0, const kf_icp_params* p, const kf_frame_maps* live,
+                   const kf_frame_maps* model, const double* guess, double* out, kf_icp_stats* st) {
+  Maps<float> L, M;
+  import_maps(live, L, true);
+  import_maps(model, M, false);
+  icp_track<float>(*k0, *p, L, M, guess, out, st);
+}
+
+void kfo_constant_velocity(const double* T2, const double* T1, double* G) {
+  double Ti[16], Dl[16];
+  rigid_inverse(T2, Ti);
+  mat4_mul(Ti, T1, Dl);
+  mat4_mul(T1, Dl, G);
+}
+
+// Pipelines: precision 0 = FP32 (volume format from the desc), 1 = FP64.
+void* kfo_pipeline_create(const kf_pipeline_desc* d, int32_t precision) {
+  PipelineBase* p = precision == 0 ? (PipelineBase*)new Pipeline<float>() : (PipelineBase*)new Pipeline<double>();
+  p->d = *d;
+  p->vd = to_vd(d->volume, precision == 0 ? d->volume.format : kFmtF64);
+  const size_t N = (size_t)d->volume.n * d->volume.n * d->volume.n;
+  p->vox.resize(N * voxel_bytes(p->vd.format));
+  memset(&p->stats, 0, sizeof(p->stats));
+  return p;
+}
+void kfo_pipeline_destroy(void* h) { delete (PipelineBase*)h; }
+void kfo_pipeline_reset(void* h, const double* T0) {
+  PipelineBase* p = (PipelineBase*)h;
+  volume_reset(p->vd, p->vox.data());
+  memcpy(p->Tcur, T0, sizeof(p->Tcur));
+  memcpy(p->Tprev1, T0, sizeof(p->Tcur));
+  memcpy(p->Tprev2, T0, sizeof(p->Tcur));
+  p->frame = 0;
+}
+void kfo_pipeline_process(void* h, const uint16_t* depth) { ((PipelineBase*)h)->process(depth); }
+void kfo_pipeline_get_pose(void* h, double* T) { memcpy(T, ((PipelineBase*)h)->Tcur, 16 * sizeof(double)); }
+void kfo_pipeline_get_stats(void* h, kf_icp_stats* s) { *s = ((PipelineBase*)h)->stats; }
+void kfo_pipeline_stage_ms(void* h, double* ms) { memcpy(ms, ((PipelineBase*)h)->stage_ms, 4 * sizeof(double)); }
+void* kfo_pipeline_volume(void* h, size_t* bytes) {
+  PipelineBase* p = (PipelineBase*)h;
+  *bytes = p->vox.size();
+  return p->vox.data();
+}
+
+}  // extern "C"
