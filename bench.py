@@ -8,8 +8,10 @@ a 512^3 volume over a 4 m cube (T = 0.02 m).  A step is one frame: depth frame
 ICP (10/5/4 iterations, FP64 solve on the device) -> TSDF sweep -> raycast.
 Frames 0..W-1 warm up, frames W..W+K-1 are timed.
 
-Variants on the same stream: plain FP32 volume (8 B/voxel) and packed FP16+u16
-volume (4 B/voxel); `value` is the FP16 pipeline's fps.  A stage-timing pass
+Variants on the same stream: plain FP32 volume (8 B/voxel), packed FP16+u16
+volume (4 B/voxel), and FP16 + the learned pose prior (ICP warm-started with
+4/3/3 iterations, plain 10/5/4 fallback) when trained weights are present;
+`value` is the fps of the last of these that ran (learned + FP16 normally).  A stage-timing pass
 (hipEvents on the pipeline's stream) gives per-kernel times, the sweep's
 algorithmic bytes (U*2*B_v + P*4 with U from the sweep's own update counter) and
 its roofline fraction.  The CPU baseline runs the oracle (OpenMP restatement of
@@ -51,9 +53,19 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def timed_run(kf, synth, metrics, stream, dd, preset: str, W: int, K: int) -> dict:
+def attach_prior(kf, p):
+    from openfusion_amd import prior_train
+    prior = kf.Prior(max_batch=1)
+    prior_train.load(prior, prior_train.DEFAULT_WEIGHTS)
+    p.attach_prior(prior)
+    return prior
+
+
+def timed_run(kf, synth, metrics, stream, dd, preset: str, W: int, K: int, use_prior: bool = False) -> dict:
     desc = synth.pipeline_desc(preset)
     p = kf.Pipeline(desc)
+    if use_prior:
+        attach_prior(kf, p)
     fb = stream.depth[0].nbytes
     p.reset(stream.poses[0])
     for i in range(W):
@@ -80,7 +92,12 @@ def timed_run(kf, synth, metrics, stream, dd, preset: str, W: int, K: int) -> di
     n = min(traj.shape[0], W + K)
     st = p.stats()
     p.close()
+    extra = {}
+    if use_prior:
+        extra = {"prior_frames": int(st.prior_frames), "prior_fallbacks": int(st.prior_fallbacks),
+                 "fallback_rate": float(st.prior_fallbacks) / max(1, st.prior_frames)}
     return {
+        **extra,
         "fps": K / wall,
         "ms_per_frame": 1000.0 * wall / K,
         "gpu_ms_per_frame": ms.value / K,
@@ -91,11 +108,13 @@ def timed_run(kf, synth, metrics, stream, dd, preset: str, W: int, K: int) -> di
     }
 
 
-def pcie_inclusive_fps(kf, synth, stream, preset: str, W: int, K: int) -> float:
+def pcie_inclusive_fps(kf, synth, stream, preset: str, W: int, K: int, use_prior: bool = False) -> float:
     """Same pipeline, each frame first copied from pinned host memory (H2D)."""
     import ctypes as C
     desc = synth.pipeline_desc(preset)
     p = kf.Pipeline(desc)
+    if use_prior:
+        attach_prior(kf, p)
     L = kf.lib()
     fb = stream.depth[0].nbytes
     hp = C.c_void_p()
@@ -119,9 +138,11 @@ def pcie_inclusive_fps(kf, synth, stream, preset: str, W: int, K: int) -> float:
     return K / wall
 
 
-def stage_run(kf, synth, stream, dd, preset: str, W: int, S: int) -> dict:
+def stage_run(kf, synth, stream, dd, preset: str, W: int, S: int, use_prior: bool = False) -> dict:
     desc = synth.pipeline_desc(preset)
     p = kf.Pipeline(desc)
+    if use_prior:
+        attach_prior(kf, p)
     p.set_stage_timing(True)
     fb = stream.depth[0].nbytes
     p.reset(stream.poses[0])
@@ -131,11 +152,12 @@ def stage_run(kf, synth, stream, dd, preset: str, W: int, S: int) -> dict:
         if i >= W:
             st = p.stats()
             rows.append((p.stage_ms(), st.updated_voxels, st.raycast_steps))
+    vstats = p.volume().stats()  # one full read of the volume (also a known-size read for counters)
     p.close()
     ms = np.array([r[0] for r in rows])
     U = np.array([r[1] for r in rows], dtype=np.float64)
     steps = np.array([r[2] for r in rows], dtype=np.float64)
-    return {"ms": ms, "U": U, "steps": steps, "desc": desc}
+    return {"ms": ms, "U": U, "steps": steps, "desc": desc, "observed_voxels": float(vstats[0])}
 
 
 def cpu_baseline(orc, synth, stream, preset: str, frames: int) -> dict:
@@ -165,7 +187,8 @@ def main() -> int:
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames of the bounded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
-    ap.add_argument("--only", choices=["plain", "fp16"], default=None)
+    ap.add_argument("--only", choices=["plain", "fp16", "learned_fp16"], default=None)
+    ap.add_argument("--no-prior", action="store_true", help="skip the learned-prior variant")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
@@ -183,18 +206,24 @@ def main() -> int:
     log(f"stream: {stream.depth.shape[0]} frames, {stream.scene.count} primitives ({time.perf_counter()-t:.1f}s)")
     dd = kf.upload(stream.depth)
 
+    from openfusion_amd import prior_train
+    have_prior = os.path.exists(prior_train.DEFAULT_WEIGHTS) and not args.no_prior
+    variants = [("plain", "full", False), ("fp16", "full16", False)]
+    if have_prior:
+        variants.append(("learned_fp16", "full16", True))
     res = {}
-    for name, preset in (("plain", "full"), ("fp16", "full16")):
+    for name, preset, use_prior in variants:
         if args.only and args.only != name:
             continue
-        log(f"timed run {name} ({preset}): warmup {W}, steps {K}")
-        res[name] = timed_run(kf, synth, metrics, stream, dd, preset, W, K)
+        log(f"timed run {name} ({preset}{' + prior' if use_prior else ''}): warmup {W}, steps {K}")
+        res[name] = timed_run(kf, synth, metrics, stream, dd, preset, W, K, use_prior)
         log(f"  {name}: {res[name]['fps']:.1f} fps, ATE {res[name]['ate_mm']:.3f} mm")
 
-    head = "fp16" if "fp16" in res else "plain"
-    head_preset = "full16" if head == "fp16" else "full"
-    log(f"stage timing ({head_preset}, {args.stage_frames} frames)")
-    sr = stage_run(kf, synth, stream, dd, head_preset, W, args.stage_frames)
+    head = "learned_fp16" if "learned_fp16" in res else ("fp16" if "fp16" in res else "plain")
+    head_preset = "full" if head == "plain" else "full16"
+    head_prior = head == "learned_fp16"
+    log(f"stage timing ({head}, {args.stage_frames} frames)")
+    sr = stage_run(kf, synth, stream, dd, head_preset, W, args.stage_frames, head_prior)
     d = sr["desc"]
     n = d.volume.n
     Bv = 4 if d.volume.format == kf.KF_VOXEL_F16 else 8
@@ -222,10 +251,12 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"{n}^3 {'FP16+u16' if Bv == 4 else 'FP32+FP32'} TSDF over 4 m (T 0.02 m), 640x480, "
-                        f"seeded 300-frame synthetic stream, ICP 3 levels 10/5/4, constant-velocity guess",
+                        f"seeded 300-frame synthetic stream, ICP 3 levels "
+                        + ("4/3/3 warm-started by the learned prior (fallback: plain 10/5/4)" if head_prior else
+                           "10/5/4 from the constant-velocity guess"),
             "volume_format": "f16+u16" if Bv == 4 else "f32+f32",
             "frames_timed": f"{W}..{W + K - 1}",
-            "prior": "none (learned prior not attached in this run)",
+            "variant": head,
         },
         "roofline": {
             "kernel": "integrate (TSDF sweep)",
@@ -242,17 +273,21 @@ def main() -> int:
         },
         "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in res.items()},
+        "observed_voxels_after_stage_run": sr["observed_voxels"],
         "stages_ms": dict(zip(["preprocess", "icp", "column_spans", "sweep", "raycast"],
                               [round(float(x), 4) for x in sr["ms"].mean(0)])),
         "raycast": {"mean_steps_per_ray": round(S_bar, 2), "algorithmic_GBs":
                     round(ray_bytes / (ray_ms.mean() * 1e-3) / 1e9, 1)},
     }
-    if "plain" in res and "fp16" in res:
-        out["speedup_fp16_vs_plain"] = round(res["fp16"]["fps"] / res["plain"]["fps"], 3)
-        out["ate_mm"] = round(res["fp16"]["ate_mm"], 3)
+    if "plain" in res:
+        for name in ("fp16", "learned_fp16"):
+            if name in res:
+                out[f"speedup_{name}_vs_plain"] = round(res[name]["fps"] / res["plain"]["fps"], 3)
+        out["ate_mm"] = round(res[head]["ate_mm"], 3)
         out["ate_plain_mm"] = round(res["plain"]["ate_mm"], 3)
     if not args.no_pcie:
-        out["fps_pcie_inclusive"] = round(pcie_inclusive_fps(kf, synth, stream, head_preset, W, min(K, 100)), 2)
+        out["fps_pcie_inclusive"] = round(pcie_inclusive_fps(kf, synth, stream, head_preset, W, min(K, 100),
+                                                             head_prior), 2)
     if not args.no_cpu:
         from oracle import oracle as orc
         log(f"cpu baseline: oracle FP32 pipeline, {args.cpu_frames} frames at 512^3")

@@ -256,6 +256,45 @@ kf_status kf_pose_constant_velocity(const kf_pose* d_prev2, const kf_pose* d_pre
                                     kf_pose* d_guess, kf_stream s);
 
 /* ------------------------------------------------------------------------ */
+/* Learned pose prior (no counterpart in the reference, whose ICP starts from the
+ * previous pose, Source/src/ICP.cpp:94).  A CNN reads two consecutive 160x120
+ * depth maps (metres, 0 = invalid) and the previous frame's motion and predicts
+ * the current frame's motion, which warm-starts ICP.
+ *   conv 3x3/s2 2->32, 32->64, 64->64, 64->128 (ReLU) -> global average pool ->
+ *   concat(previous twist) -> FC 134->256 (ReLU) -> FC 256->6 -> + previous twist.
+ * Every product is a hand-written BF16 MFMA GEMM (FP32 accumulate); weights are
+ * FP32 masters with BF16 copies; training is Adam on the device.
+ * Twist: (w, t) with R = exp([w]x) (rotation vector) and t the translation of the
+ * motion D = T_prev^-1 T_cur, scaled by 1/KF_PRIOR_ROT_SCALE and 1/KF_PRIOR_TRANS_SCALE.
+ * Parameter blob (FP32): per layer W (row = output channel; conv columns ordered
+ * (ky, kx, cin)) then b, for conv1..conv4, fc1, fc2. */
+#define KF_PRIOR_WIDTH 160
+#define KF_PRIOR_HEIGHT 120
+#define KF_PRIOR_ROT_SCALE 0.05   /* rad  */
+#define KF_PRIOR_TRANS_SCALE 0.05 /* metre */
+
+typedef struct kf_prior kf_prior;
+
+int64_t kf_prior_param_count(void);
+kf_status kf_prior_create(int32_t max_batch, kf_prior** out);
+kf_status kf_prior_destroy(kf_prior* p);
+kf_status kf_prior_init_params(kf_prior* p, uint64_t seed); /* He-uniform init, zero biases; resets Adam */
+kf_status kf_prior_set_params(kf_prior* p, const float* h_params, int64_t count); /* blocking */
+kf_status kf_prior_get_params(kf_prior* p, float* h_params, int64_t count);       /* blocking */
+/* Batched forward.  d_frames: [F][120][160] float depth maps; d_pairs: [batch][2]
+ * int32 frame indices (previous, current); d_prev_twist, d_out: [batch][6] scaled twists. */
+kf_status kf_prior_forward(kf_prior* p, const float* d_frames, const int32_t* d_pairs, const float* d_prev_twist,
+                           int32_t batch, float* d_out, kf_stream s);
+/* Forward + backward of the loss mean((out - target)^2) over batch*6 outputs;
+ * writes dLoss/dparams (FP32, param_count) to d_grad and the loss to d_loss (nullable). */
+kf_status kf_prior_gradients(kf_prior* p, const float* d_frames, const int32_t* d_pairs, const float* d_prev_twist,
+                             const float* d_target, int32_t batch, float* d_grad, float* d_loss, kf_stream s);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on that loss; d_loss nullable. */
+kf_status kf_prior_train_step(kf_prior* p, const float* d_frames, const int32_t* d_pairs,
+                              const float* d_prev_twist, const float* d_target, int32_t batch, float lr,
+                              float* d_loss, kf_stream s);
+
+/* ------------------------------------------------------------------------ */
 /* Frame pipeline: the per-frame driver of RunApp::display
  * (Source/src/MainApp.cpp:774-970) in KinectFusion order
  * preprocess -> ICP (guess: constant velocity or learned prior) -> integrate -> raycast,
@@ -282,6 +321,8 @@ typedef struct kf_frame_stats {
   kf_icp_stats icp_prior;    /* warm-started ICP (valid if used_prior) */
   unsigned long long updated_voxels; /* stage-timing mode only: voxels updated by the sweep */
   unsigned long long raycast_steps;  /* stage-timing mode only: total march steps */
+  int32_t prior_frames;      /* frames tracked with the prior since the last reset */
+  int32_t prior_fallbacks;   /* of which fell back to plain ICP */
 } kf_frame_stats;
 
 typedef struct kf_pipeline kf_pipeline;
@@ -294,6 +335,13 @@ kf_status kf_pipeline_destroy(kf_pipeline* p);
 kf_status kf_pipeline_reset(kf_pipeline* p, const double T0_wc[16]);
 /* Processes one uint16 depth frame already in device memory. */
 kf_status kf_pipeline_process(kf_pipeline* p, const uint16_t* d_depth);
+/* Attaches a learned prior (not owned; NULL detaches) before the next tracked frame.
+ * Each frame then: prior guess -> ICP with icp_prior iterations -> accept, or fall
+ * back to plain ICP from the constant-velocity guess when that ICP failed, its
+ * residual exceeds fallback_residual_ratio x the running median of the last 31
+ * accepted residuals, or its inliers / candidates < fallback_min_inlier_frac.
+ * Requires a 640x480 level 0 with 3 levels (the prior reads the 160x120 level). */
+kf_status kf_pipeline_attach_prior(kf_pipeline* p, kf_prior* prior);
 kf_status kf_pipeline_get_pose(kf_pipeline* p, double T_wc[16]);          /* blocking */
 /* Poses of every frame since the last reset (device log, up to KF_TRAJ_CAPACITY
  * frames, written by the pipeline itself): copies min(frames, max_frames) T_wc

@@ -286,6 +286,7 @@ __global__ void pose_const_velocity(const kf_pose* p2, const kf_pose* p1, kf_pos
   rigid_inverse(p2->T_wc, Ti);
   mat4_mul(Ti, p1->T_wc, Dl);
   mat4_mul(p1->T_wc, Dl, G);
+  orthonormalize(G);
   kf_pose np;
   for (int q = 0; q < 16; ++q) np.T_wc[q] = G[q];
   pose_finish_fields(&np);

@@ -8,6 +8,7 @@
 //  * nothing returns to the host inside a frame: poses, stats and the ICP solve
 //    stay on the device, and frames after the first tracked one replay a hipGraph.
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <string.h>
 
 #include "kf_internal.hpp"
@@ -15,7 +16,24 @@
 
 namespace {
 
-enum PoseSlot { kCur = 0, kPrev1 = 1, kPrev2 = 2, kGuess = 3, kSlots = 4 };
+enum PoseSlot { kCur = 0, kPrev1 = 1, kPrev2 = 2, kGuess = 3, kGuessPrior = 4, kPriorOut = 5, kSlots = 6 };
+
+constexpr int kWindow = 31;      // accepted residuals kept for the running median
+constexpr int kWindowMin = 5;    // residual test starts once this many are known
+
+// Device-side state of the learned prior in the pipeline.
+struct PriorState {
+  float prev_twist[6];  // scaled twist of the previous frame's motion (network input)
+  float out_twist[6];   // network output
+  int32_t pairs[2];     // {0, 1}: slot 0 = previous 160x120 depth, slot 1 = current
+  int32_t used_prior;
+  int32_t fell_back;
+  int32_t win_count;
+  int32_t win_head;
+  int32_t fallbacks;
+  int32_t frames;
+  double window[kWindow];
+};
 
 }  // namespace
 
@@ -38,6 +56,10 @@ struct kf_pipeline {
   unsigned long long* d_counters = nullptr;  // [updated voxels, raycast steps] (stage-timing mode)
   double* d_traj = nullptr;                  // KF_TRAJ_CAPACITY poses (16 doubles each)
   int* d_traj_count = nullptr;
+  kf_prior* prior = nullptr;                 // attached learned prior (not owned)
+  kf_tracker* tracker_prior = nullptr;       // warm-started ICP (icp_prior iterations)
+  PriorState* d_prior = nullptr;
+  float* d_prior_frames = nullptr;           // [2][120][160]
 };
 
 namespace kf {
@@ -56,6 +78,156 @@ __global__ void pose_shift(kf_pose* poses, double* traj, int* count) {
   poses[kPrev1] = poses[kCur];
 }
 
+// slot 0 <- slot 1 (previous frame's 160x120 depth), slot 1 <- current level-2 depth.
+__global__ void prior_stage_inputs(const float* l2, float* frames, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  frames[i] = frames[n + i];
+  frames[n + i] = l2[i];
+}
+
+__device__ void rotvec_to_R(const double* w, double* R) {
+  const double th2 = (w[0] * w[0] + w[1] * w[1]) + w[2] * w[2];
+  const double th = sqrt(th2);
+  double a, b;  // sin(th)/th, (1 - cos(th))/th^2
+  if (th < 1e-6) {
+    a = 1.0 - th2 / 6.0;
+    b = 0.5 - th2 / 24.0;
+  } else {
+    a = sin(th) / th;
+    b = (1.0 - cos(th)) / th2;
+  }
+  const double K[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double k2 = 0.0;
+      for (int m = 0; m < 3; ++m) k2 += K[r * 3 + m] * K[m * 3 + c];
+      R[r * 3 + c] = (r == c ? 1.0 : 0.0) + a * K[r * 3 + c] + b * k2;
+    }
+}
+
+__device__ void R_to_rotvec(const double* R, double* w) {
+  const double tr = (R[0] + R[4]) + R[8];
+  const double c = fmin(1.0, fmax(-1.0, 0.5 * (tr - 1.0)));
+  const double th = acos(c);
+  const double v[3] = {R[7] - R[5], R[2] - R[6], R[3] - R[1]};
+  const double s = th < 1e-6 ? 0.5 : th / (2.0 * sin(th));
+  for (int i = 0; i < 3; ++i) w[i] = s * v[i];
+}
+
+// guess = T_{k-1} * [exp(w) | t] from the network's scaled twist; the
+// constant-velocity guess if the output is not finite.
+__global__ void prior_guess(PriorState* st, kf_pose* poses) {
+  if (threadIdx.x != 0) return;
+  bool ok = true;
+  double w[3], t[3];
+  for (int i = 0; i < 3; ++i) {
+    w[i] = (double)st->out_twist[i] * KF_PRIOR_ROT_SCALE;
+    t[i] = (double)st->out_twist[3 + i] * KF_PRIOR_TRANS_SCALE;
+    ok = ok && isfinite(w[i]) && isfinite(t[i]);
+  }
+  if (!ok) {
+    poses[kGuessPrior] = poses[kGuess];
+    return;
+  }
+  double R[9], D[16], G[16];
+  rotvec_to_R(w, R);
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) D[r * 4 + c] = R[r * 3 + c];
+    D[r * 4 + 3] = t[r];
+  }
+  D[12] = D[13] = D[14] = 0.0;
+  D[15] = 1.0;
+  mat4_mul(poses[kPrev1].T_wc, D, G);
+  orthonormalize(G);
+  kf_pose np;
+  for (int q = 0; q < 16; ++q) np.T_wc[q] = G[q];
+  pose_finish_fields(&np);
+  poses[kGuessPrior] = np;
+}
+
+// Accept the warm-started ICP or arm the plain ICP (its skip flag).
+__global__ void prior_decide(PriorState* st, const kf_icp_stats* sp, kf_icp_stats* plain, kf_pose* poses,
+                             float ratio, float min_frac) {
+  if (threadIdx.x != 0) return;
+  bool fail = sp->status != 0 || sp->iterations == 0 || !(sp->residual_rms == sp->residual_rms);
+  if (!fail && sp->candidates > 0 && (double)sp->inliers < (double)min_frac * (double)sp->candidates) fail = true;
+  if (!fail && st->win_count >= kWindowMin) {
+    double v[kWindow];
+    const int n = st->win_count;
+    for (int i = 0; i < n; ++i) v[i] = st->window[i];
+    for (int i = 1; i < n; ++i) {
+      const double x = v[i];
+      int j = i - 1;
+      while (j >= 0 && v[j] > x) {
+        v[j + 1] = v[j];
+        --j;
+      }
+      v[j + 1] = x;
+    }
+    const double med = (n & 1) ? v[n / 2] : 0.5 * (v[n / 2 - 1] + v[n / 2]);
+    if (sp->residual_rms > (double)ratio * med) fail = true;
+  }
+  st->used_prior = 1;
+  st->fell_back = fail ? 1 : 0;
+  if (!fail) poses[kCur] = poses[kPriorOut];
+  plain->skip = fail ? 0 : 1;
+}
+
+// Frame end with a prior: record the accepted residual and the network's next
+// motion input (scaled twist of T_{k-1}^-1 T_k).
+__global__ void prior_post(PriorState* st, const kf_icp_stats* sp, const kf_icp_stats* plain,
+                           const kf_pose* poses) {
+  if (threadIdx.x != 0) return;
+  const double res = st->fell_back ? plain->residual_rms : sp->residual_rms;
+  if (res == res && res > 0.0) {
+    st->window[st->win_head] = res;
+    st->win_head = (st->win_head + 1) % kWindow;
+    if (st->win_count < kWindow) st->win_count += 1;
+  }
+  st->fallbacks += st->fell_back;
+  st->frames += 1;
+  double Ti[16], D[16];
+  rigid_inverse(poses[kPrev1].T_wc, Ti);
+  mat4_mul(Ti, poses[kCur].T_wc, D);
+  const double R[9] = {D[0], D[1], D[2], D[4], D[5], D[6], D[8], D[9], D[10]};
+  double w[3];
+  R_to_rotvec(R, w);
+  for (int i = 0; i < 3; ++i) {
+    st->prev_twist[i] = (float)(w[i] / KF_PRIOR_ROT_SCALE);
+    st->prev_twist[3 + i] = (float)(D[i * 4 + 3] / KF_PRIOR_TRANS_SCALE);
+  }
+}
+
+kf_status stage_prior_inputs(kf_pipeline* p, hipStream_t s) {
+  const int n = KF_PRIOR_WIDTH * KF_PRIOR_HEIGHT;
+  hipLaunchKernelGGL(prior_stage_inputs, dim3(div_up(n, 256)), dim3(256), 0, s,
+                     (const float*)p->live.level[2].depth, p->d_prior_frames, n);
+  return check_launch("prior_stage_inputs");
+}
+
+kf_status record_track_prior(kf_pipeline* p, hipStream_t s) {
+  const kf_pipeline_desc& d = p->d;
+  KF_TRY(stage_prior_inputs(p, s));
+  KF_TRY(launch_pose_constant_velocity(&p->d_poses[kPrev2], &p->d_poses[kPrev1], &p->d_poses[kGuess], s));
+  char* st = reinterpret_cast<char*>(p->d_prior);  // device addresses of PriorState members
+  KF_TRY(kf_prior_forward(p->prior, p->d_prior_frames, reinterpret_cast<int32_t*>(st + offsetof(PriorState, pairs)),
+                          reinterpret_cast<float*>(st + offsetof(PriorState, prev_twist)), 1,
+                          reinterpret_cast<float*>(st + offsetof(PriorState, out_twist)), s));
+  hipLaunchKernelGGL(prior_guess, dim3(1), dim3(64), 0, s, p->d_prior, p->d_poses);
+  KF_TRY(check_launch("prior_guess"));
+  KF_TRY(launch_icp_track(p->tracker_prior, p->live, p->model, &p->d_poses[kGuessPrior], &p->d_poses[kPriorOut],
+                          &p->d_stats[1], s));
+  hipLaunchKernelGGL(prior_decide, dim3(1), dim3(64), 0, s, p->d_prior, (const kf_icp_stats*)&p->d_stats[1],
+                     &p->d_stats[0], p->d_poses, d.fallback_residual_ratio, d.fallback_min_inlier_frac);
+  KF_TRY(check_launch("prior_decide"));
+  KF_TRY(launch_icp_track(p->tracker, p->live, p->model, &p->d_poses[kGuess], &p->d_poses[kCur], &p->d_stats[0],
+                          s));
+  hipLaunchKernelGGL(prior_post, dim3(1), dim3(64), 0, s, p->d_prior, (const kf_icp_stats*)&p->d_stats[1],
+                     (const kf_icp_stats*)&p->d_stats[0], (const kf_pose*)p->d_poses);
+  return check_launch("prior_post");
+}
+
 // One tracked frame.  In stage-timing mode (eager only) events bracket each stage
 // and the sweep kernel alone, and the sweep / raycast counters are enabled.
 kf_status record_track_frame(kf_pipeline* p, bool timed) {
@@ -69,9 +241,13 @@ kf_status record_track_frame(kf_pipeline* p, bool timed) {
   }
   KF_TRY(launch_preprocess(d.intr, d.pre, p->d_depth, p->live, s));
   if (timed) KF_HIP(hipEventRecord(p->ev[1], s));
-  KF_TRY(launch_pose_constant_velocity(&p->d_poses[kPrev2], &p->d_poses[kPrev1], &p->d_poses[kGuess], s));
-  KF_TRY(launch_icp_track(p->tracker, p->live, p->model, &p->d_poses[kGuess], &p->d_poses[kCur],
-                          &p->d_stats[0], s));
+  if (p->prior) {
+    KF_TRY(record_track_prior(p, s));
+  } else {
+    KF_TRY(launch_pose_constant_velocity(&p->d_poses[kPrev2], &p->d_poses[kPrev1], &p->d_poses[kGuess], s));
+    KF_TRY(launch_icp_track(p->tracker, p->live, p->model, &p->d_poses[kGuess], &p->d_poses[kCur],
+                            &p->d_stats[0], s));
+  }
   if (timed) KF_HIP(hipEventRecord(p->ev[2], s));
   KF_TRY(launch_integrate(p->vol, d.intr, d.integ, p->live.raylen, &p->d_poses[kCur], upd, nullptr, s,
                           timed ? p->ev[3] : nullptr));
@@ -87,10 +263,21 @@ kf_status bootstrap_frame(kf_pipeline* p) {
   hipStream_t s = p->stream;
   const kf_pipeline_desc& d = p->d;
   KF_TRY(launch_preprocess(d.intr, d.pre, p->d_depth, p->live, s));
+  if (p->prior) KF_TRY(stage_prior_inputs(p, s));
   KF_TRY(launch_integrate(p->vol, d.intr, d.integ, p->live.raylen, &p->d_poses[kCur], nullptr, nullptr, s));
   KF_TRY(launch_raycast(p->vol, d.intr, d.ray, &p->d_poses[kCur], p->model, nullptr, s));
   hipLaunchKernelGGL(pose_shift, dim3(1), dim3(64), 0, s, p->d_poses, p->d_traj, p->d_traj_count);
   return check_launch("pose_shift");
+}
+
+kf_status reset_prior_state(kf_pipeline* p) {
+  PriorState h;
+  memset(&h, 0, sizeof(h));
+  h.pairs[0] = 0;
+  h.pairs[1] = 1;
+  KF_HIP(hipMemcpyAsync(p->d_prior, &h, sizeof(h), hipMemcpyHostToDevice, p->stream));
+  KF_HIP(hipStreamSynchronize(p->stream));
+  return KF_OK;
 }
 
 void destroy_graph(kf_pipeline* p) {
@@ -157,6 +344,9 @@ extern "C" kf_status kf_pipeline_destroy(kf_pipeline* p) {
   (void)hipFree(p->d_traj_count);
   kf_volume_destroy(p->vol);
   kf_tracker_destroy(p->tracker);
+  kf_tracker_destroy(p->tracker_prior);
+  (void)hipFree(p->d_prior);
+  (void)hipFree(p->d_prior_frames);
   if (p->live.level[0].vx) kf_maps_free(&p->live);
   if (p->model.level[0].vx) kf_maps_free(&p->model);
   (void)hipFree(p->d_poses);
@@ -215,6 +405,7 @@ extern "C" kf_status kf_pipeline_reset(kf_pipeline* p, const double T0[16]) {
   for (int i = 0; i < kSlots; ++i) KF_TRY(kf_pose_upload(&p->d_poses[i], T0, p->stream));
   KF_HIP(hipMemsetAsync(p->d_stats, 0, 2 * sizeof(kf_icp_stats), p->stream));
   KF_HIP(hipMemsetAsync(p->d_traj_count, 0, sizeof(int), p->stream));
+  if (p->d_prior) KF_TRY(reset_prior_state(p));
   KF_HIP(hipStreamSynchronize(p->stream));
   p->frame = 0;
   return KF_OK;
@@ -263,6 +454,24 @@ extern "C" kf_status kf_pipeline_get_pose(kf_pipeline* p, double T[16]) {
   return kf_pose_download(T, &p->d_poses[kCur], p->stream);
 }
 
+extern "C" kf_status kf_pipeline_attach_prior(kf_pipeline* p, kf_prior* prior) {
+  KF_CHECK_ARG(p != nullptr, "pipeline is NULL");
+  KF_HIP(hipStreamSynchronize(p->stream));
+  destroy_graph(p);  // the recorded frame changes
+  p->prior = prior;
+  if (!prior) return KF_OK;
+  const kf_pipeline_desc& d = p->d;
+  KF_CHECK_ARG(d.icp.levels == 3 && d.intr.width == 4 * KF_PRIOR_WIDTH && d.intr.height == 4 * KF_PRIOR_HEIGHT,
+               "the prior needs a 640x480 level 0 and 3 pyramid levels");
+  if (!p->tracker_prior) KF_TRY(kf_tracker_create(&d.intr, &d.icp_prior, &p->tracker_prior));
+  if (!p->d_prior) {
+    KF_HIP(hipMalloc(&p->d_prior, sizeof(PriorState)));
+    KF_HIP(hipMalloc(&p->d_prior_frames, 2 * KF_PRIOR_WIDTH * KF_PRIOR_HEIGHT * sizeof(float)));
+    KF_HIP(hipMemset(p->d_prior_frames, 0, 2 * KF_PRIOR_WIDTH * KF_PRIOR_HEIGHT * sizeof(float)));
+  }
+  return reset_prior_state(p);
+}
+
 extern "C" kf_status kf_pipeline_get_trajectory(kf_pipeline* p, double* T_out, int32_t max_frames,
                                                 int32_t* frames) {
   KF_CHECK_ARG(p && frames && (T_out || max_frames == 0), "NULL argument");
@@ -290,6 +499,15 @@ extern "C" kf_status kf_pipeline_get_stats(kf_pipeline* p, kf_frame_stats* st) {
   st->frame = p->frame - 1;
   st->icp = h[0];
   st->icp_prior = h[1];
+  if (p->prior && p->d_prior) {
+    PriorState ps;
+    KF_HIP(hipMemcpy(&ps, p->d_prior, sizeof(ps), hipMemcpyDeviceToHost));
+    st->used_prior = ps.used_prior;
+    st->fell_back = ps.fell_back;
+    st->prior_frames = ps.frames;
+    st->prior_fallbacks = ps.fallbacks;
+    if (!ps.fell_back) st->icp = h[1];
+  }
   st->updated_voxels = c[0];
   st->raycast_steps = c[1];
   return KF_OK;

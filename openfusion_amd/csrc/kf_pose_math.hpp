@@ -41,6 +41,25 @@ KF_HD void mat4_mul(const double* A, const double* B, double* C) {
                      A[i * 4 + 3] * B[3 * 4 + j];
 }
 
+// One Newton step of the polar decomposition on the rotation block,
+// R <- R (3I - R^T R) / 2: removes the (quadratically small) drift from
+// orthonormality that repeated pose products accumulate.  The constant-velocity
+// prediction T1 (T0^-1 T1) would otherwise amplify it by ~2.4x per frame.
+KF_HD void orthonormalize(double* T) {
+  double S[9];  // R^T R
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      S[i * 3 + j] = (T[0 * 4 + i] * T[0 * 4 + j] + T[1 * 4 + i] * T[1 * 4 + j]) + T[2 * 4 + i] * T[2 * 4 + j];
+  double R[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      const double rs = (T[r * 4 + 0] * S[0 * 3 + c] + T[r * 4 + 1] * S[1 * 3 + c]) + T[r * 4 + 2] * S[2 * 3 + c];
+      R[r * 3 + c] = 1.5 * T[r * 4 + c] - 0.5 * rs;
+    }
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) T[r * 4 + c] = R[r * 3 + c];
+}
+
 // Fills the FP32 fields of p from p->T_wc.
 KF_HD void pose_finish_fields(kf_pose* p) {
   double Ti[16];

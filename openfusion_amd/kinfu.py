@@ -100,7 +100,8 @@ class PipelineDesc(C.Structure):
 class FrameStats(C.Structure):
     _fields_ = [("frame", C.c_int32), ("used_prior", C.c_int32), ("fell_back", C.c_int32),
                 ("pad_", C.c_int32), ("icp", IcpStats), ("icp_prior", IcpStats),
-                ("updated_voxels", C.c_uint64), ("raycast_steps", C.c_uint64)]
+                ("updated_voxels", C.c_uint64), ("raycast_steps", C.c_uint64),
+                ("prior_frames", C.c_int32), ("prior_fallbacks", C.c_int32)]
 
 
 STAGES = 5  # preprocess, guess+icp, column spans, sweep kernel, raycast
@@ -153,11 +154,21 @@ _PROTOS = {
     "kf_icp_reduce": (C.c_int, [_VP, C.c_int32, C.POINTER(FrameMaps), C.POINTER(FrameMaps), _VP, _VP, _VP,
                                 _VP]),
     "kf_pose_constant_velocity": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "kf_prior_param_count": (C.c_int64, []),
+    "kf_prior_create": (C.c_int, [C.c_int32, C.POINTER(_VP)]),
+    "kf_prior_destroy": (C.c_int, [_VP]),
+    "kf_prior_init_params": (C.c_int, [_VP, C.c_uint64]),
+    "kf_prior_set_params": (C.c_int, [_VP, C.POINTER(C.c_float), C.c_int64]),
+    "kf_prior_get_params": (C.c_int, [_VP, C.POINTER(C.c_float), C.c_int64]),
+    "kf_prior_forward": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, _VP, _VP]),
+    "kf_prior_gradients": (C.c_int, [_VP, _VP, _VP, _VP, _VP, C.c_int32, _VP, _VP, _VP]),
+    "kf_prior_train_step": (C.c_int, [_VP, _VP, _VP, _VP, _VP, C.c_int32, C.c_float, _VP, _VP]),
     "kf_pipeline_default_desc": (C.c_int, [C.POINTER(PipelineDesc)]),
     "kf_pipeline_create": (C.c_int, [C.POINTER(PipelineDesc), _VP, C.POINTER(_VP)]),
     "kf_pipeline_destroy": (C.c_int, [_VP]),
     "kf_pipeline_reset": (C.c_int, [_VP, C.POINTER(C.c_double)]),
     "kf_pipeline_process": (C.c_int, [_VP, _VP]),
+    "kf_pipeline_attach_prior": (C.c_int, [_VP, _VP]),
     "kf_pipeline_get_pose": (C.c_int, [_VP, C.POINTER(C.c_double)]),
     "kf_pipeline_get_stats": (C.c_int, [_VP, C.POINTER(FrameStats)]),
     "kf_pipeline_get_trajectory": (C.c_int, [_VP, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32)]),
@@ -570,6 +581,61 @@ class ICP:
 
 
 # ---------------------------------------------------------------------------
+# Learned pose prior.
+
+PRIOR_H, PRIOR_W = 120, 160
+PRIOR_ROT_SCALE = 0.05
+PRIOR_TRANS_SCALE = 0.05
+
+
+class Prior:
+    """kf_prior: the CNN pose prior (BF16 MFMA forward/backward, Adam on the device)."""
+
+    def __init__(self, max_batch: int = 256):
+        h = _VP()
+        check(lib().kf_prior_create(max_batch, C.byref(h)), "kf_prior_create")
+        self.handle = h.value
+        self.max_batch = max_batch
+        self.nparams = int(lib().kf_prior_param_count())
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().kf_prior_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, seed: int) -> None:
+        check(lib().kf_prior_init_params(self.handle, seed), "kf_prior_init_params")
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().kf_prior_set_params(self.handle, p.ctypes.data_as(C.POINTER(C.c_float)), p.size), "set_params")
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.nparams, dtype=np.float32)
+        check(lib().kf_prior_get_params(self.handle, p.ctypes.data_as(C.POINTER(C.c_float)), p.size), "get_params")
+        return p
+
+    def forward(self, d_frames, d_pairs, d_prev, batch: int, d_out, stream=None) -> None:
+        check(lib().kf_prior_forward(self.handle, _ptr(d_frames), _ptr(d_pairs), _ptr(d_prev), batch, _ptr(d_out),
+                                     stream), "kf_prior_forward")
+
+    def gradients(self, d_frames, d_pairs, d_prev, d_target, batch: int, d_grad, d_loss=None, stream=None) -> None:
+        check(lib().kf_prior_gradients(self.handle, _ptr(d_frames), _ptr(d_pairs), _ptr(d_prev), _ptr(d_target),
+                                       batch, _ptr(d_grad), _ptr(d_loss), stream), "kf_prior_gradients")
+
+    def train_step(self, d_frames, d_pairs, d_prev, d_target, batch: int, lr: float, d_loss=None,
+                   stream=None) -> None:
+        check(lib().kf_prior_train_step(self.handle, _ptr(d_frames), _ptr(d_pairs), _ptr(d_prev), _ptr(d_target),
+                                        batch, lr, _ptr(d_loss), stream), "kf_prior_train_step")
+
+
+# ---------------------------------------------------------------------------
 # Pipeline.
 
 
@@ -599,6 +665,10 @@ class Pipeline:
 
     def process(self, d_depth) -> None:
         check(lib().kf_pipeline_process(self.handle, _ptr(d_depth)), "process")
+
+    def attach_prior(self, prior: "Prior | None") -> None:
+        self._prior = prior  # keep it alive while attached
+        check(lib().kf_pipeline_attach_prior(self.handle, prior.handle if prior else None), "attach_prior")
 
     def pose(self) -> np.ndarray:
         T = np.empty(16, dtype=np.float64)

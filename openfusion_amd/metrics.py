@@ -39,9 +39,12 @@ def ate_aligned(est: np.ndarray, gt: np.ndarray) -> float:
 
 
 def rotation_error_deg(Ta: np.ndarray, Tb: np.ndarray) -> float:
+    """Angle of Ra^T Rb from atan2(|skew part|, (trace - 1) / 2), well conditioned
+    for small angles (arccos of the trace is not)."""
     R = Ta[:3, :3].T @ Tb[:3, :3]
-    c = np.clip((np.trace(R) - 1.0) / 2.0, -1.0, 1.0)
-    return float(np.degrees(np.arccos(c)))
+    s = 0.5 * np.linalg.norm([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]])
+    c = (np.trace(R) - 1.0) / 2.0
+    return float(np.degrees(np.arctan2(s, c)))
 
 
 def translation_error(Ta: np.ndarray, Tb: np.ndarray) -> float:

@@ -1,0 +1,217 @@
+"""Training of the learned pose prior on the GPU with libkinfu's own kernels.
+
+Data: 200 seeded synthetic trajectories (seeds 10000-10199, smooth random paths
+with <= 6 cm and <= 4 deg per frame, 101 frames, 6-10 primitives in a 3x3x3 m
+room) are tracked by the plain method (constant-velocity ICP 10/5/4, FP16+u16
+512^3 volume); each consecutive frame pair gives one sample: the two 160x120
+pyramid depths (metres, 0 = invalid), the previous frame's tracked motion as
+input and the tracked motion of the pair as label (twists scaled by
+1/KF_PRIOR_ROT_SCALE and 1/KF_PRIOR_TRANS_SCALE).  Pairs whose ICP failed
+(singular system, inliers below 40 % of candidates) are dropped.
+
+Training: Adam, lr 1e-3, batch 256, shuffled epochs, on the device
+(kf_prior_train_step).  The weights are written as raw little-endian FP32
+(kf_prior parameter blob) plus a JSON sidecar with the run's settings and loss
+curve.
+
+Usage: python -m openfusion_amd.prior_train [--out DIR] [--trajectories 200] [--epochs 40]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import kinfu as kf
+from . import synth
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_WEIGHTS = os.path.join(PKG, "data", "prior_weights.bin")
+P2 = kf.PRIOR_H * kf.PRIOR_W
+
+
+def rotvec(R: np.ndarray) -> np.ndarray:
+    c = np.clip((np.trace(R) - 1.0) / 2.0, -1.0, 1.0)
+    th = np.arccos(c)
+    v = np.array([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]])
+    s = 0.5 if th < 1e-6 else th / (2.0 * np.sin(th))
+    return s * v
+
+
+def twist(Ta: np.ndarray, Tb: np.ndarray) -> np.ndarray:
+    """Scaled twist (rotation vector, translation) of D = Ta^-1 Tb."""
+    D = np.linalg.inv(Ta) @ Tb
+    return np.concatenate([rotvec(D[:3, :3]) / kf.PRIOR_ROT_SCALE, D[:3, 3] / kf.PRIOR_TRANS_SCALE])
+
+
+def training_preset(i: int, frames: int, max_trans: float, max_rot: float) -> synth.Preset:
+    base = synth.PRESETS["full16"]
+    return synth.Preset(f"prior{i}", base.n, base.extent, base.trunc, base.fmt, frames, base.room,
+                        n_random=(6, 10), traj=synth.KF_TRAJ_SPLINE, max_trans=max_trans, max_rot=max_rot,
+                        scene_seed=20000 + i, traj_seed=10000 + i, noise_seed=30000 + i)
+
+
+class Dataset:
+    """Device-resident samples: depth planes [F][120][160], pairs, prev twists, targets."""
+
+    def __init__(self, planes: np.ndarray, pairs: np.ndarray, prev: np.ndarray, target: np.ndarray, gt: np.ndarray):
+        self.n = pairs.shape[0]
+        self.planes = kf.upload(planes.astype(np.float32))
+        self.pairs_h, self.prev_h, self.target_h, self.gt_h = pairs, prev, target, gt
+        self.pairs = kf.DeviceBuffer(pairs.nbytes)
+        self.prev = kf.DeviceBuffer(prev.nbytes)
+        self.target = kf.DeviceBuffer(target.nbytes)
+        self.order(np.arange(self.n))
+
+    def order(self, idx: np.ndarray) -> None:
+        self.pairs.upload(np.ascontiguousarray(self.pairs_h[idx], np.int32))
+        self.prev.upload(np.ascontiguousarray(self.prev_h[idx], np.float32))
+        self.target.upload(np.ascontiguousarray(self.target_h[idx], np.float32))
+
+
+def generate(trajectories: range, frames: int = 101, max_trans: float = 0.06, max_rot: float = 4.0,
+             log=print) -> Dataset:
+    """Tracks each trajectory with the plain GPU pipeline and collects samples."""
+    k = synth.intrinsics()
+    planes, pairs, prev, target, gt = [], [], [], [], []
+    base_plane = 0
+    t0 = time.perf_counter()
+    for ti, i in enumerate(trajectories):
+        pr = training_preset(i, frames, max_trans, max_rot)
+        s = synth.make_stream(pr)
+        p = kf.Pipeline(synth.pipeline_desc(pr))
+        dd = kf.upload(s.depth)
+        live, _ = p.maps()
+        l2 = C.cast(live.maps.level[2].depth, C.c_void_p).value
+        buf = kf.DeviceBuffer(frames * P2 * 4)
+        stream = p.stream()
+        p.reset(s.poses[0])
+        ok = np.ones(frames, dtype=bool)
+        for f in range(frames):
+            p.process(dd.ptr + f * s.depth[0].nbytes)
+            kf.check(kf.lib().kf_memcpy_dtod(buf.ptr + f * P2 * 4, l2, P2 * 4, stream), "dtod")
+            if f:
+                st = p.stats().icp
+                ok[f] = st.status == 0 and st.inliers >= 0.4 * max(st.candidates, 1)
+        kf.synchronize()
+        T = p.trajectory()
+        planes.append(buf.download(np.float32).reshape(frames, kf.PRIOR_H, kf.PRIOR_W))
+        p.close()
+        for f in range(1, frames):
+            if not ok[f] or not ok[f - 1]:
+                continue
+            pairs.append((base_plane + f - 1, base_plane + f))
+            prev.append(twist(T[f - 2], T[f - 1]) if f >= 2 else np.zeros(6))
+            target.append(twist(T[f - 1], T[f]))
+            gt.append(twist(s.poses[f - 1], s.poses[f]))
+        base_plane += frames
+        if ti % 20 == 0:
+            log(f"  trajectory {i}: {len(pairs)} samples so far ({time.perf_counter() - t0:.0f}s)")
+    return Dataset(np.concatenate(planes), np.array(pairs, np.int32), np.array(prev, np.float32),
+                   np.array(target, np.float32), np.array(gt, np.float32))
+
+
+def evaluate(prior: kf.Prior, ds: Dataset, batch: int = 256) -> dict:
+    """Mean twist error (scaled units, and mm / deg) of the prior and of constant velocity."""
+    ds.order(np.arange(ds.n))
+    out = np.zeros((ds.n, 6), np.float32)
+    dout = kf.DeviceBuffer(batch * 6 * 4)
+    for b0 in range(0, ds.n, batch):
+        nb = min(batch, ds.n - b0)
+        prior.forward(ds.planes, ds.pairs.ptr + b0 * 8, ds.prev.ptr + b0 * 24, nb, dout)
+        kf.synchronize()
+        out[b0:b0 + nb] = dout.download(np.float32, nb * 6).reshape(nb, 6)
+    tgt = ds.target_h
+
+    def err(pred):
+        d = pred - tgt
+        rot = np.linalg.norm(d[:, :3], axis=1) * kf.PRIOR_ROT_SCALE
+        tr = np.linalg.norm(d[:, 3:], axis=1) * kf.PRIOR_TRANS_SCALE
+        return {"mse": float((d ** 2).mean()), "trans_mm": float(tr.mean() * 1000), "rot_deg": float(
+            np.degrees(rot.mean()))}
+
+    return {"prior": err(out), "constant_velocity": err(ds.prev_h), "zero_motion": err(np.zeros_like(tgt))}
+
+
+def train(ds: Dataset, epochs: int = 40, batch: int = 256, lr: float = 1e-3, seed: int = 1, log=print,
+          val: Dataset | None = None) -> tuple[kf.Prior, list]:
+    prior = kf.Prior(max_batch=batch)
+    prior.init(seed)
+    rng = np.random.default_rng(seed)
+    dl = kf.DeviceBuffer(4)
+    curve = []
+    for ep in range(epochs):
+        perm = rng.permutation(ds.n)
+        ds.order(perm)
+        losses = []
+        for b0 in range(0, ds.n - batch + 1, batch):
+            prior.train_step(ds.planes, ds.pairs.ptr + b0 * 8, ds.prev.ptr + b0 * 24, ds.target.ptr + b0 * 24, batch,
+                             lr, dl)
+            kf.synchronize()
+            losses.append(float(dl.download(np.float32)[0]))
+        rec = {"epoch": ep, "train_loss": float(np.mean(losses))}
+        if val is not None and (ep % 5 == 4 or ep == epochs - 1):
+            ev = evaluate(prior, val)
+            rec["val_prior_mse"] = ev["prior"]["mse"]
+            rec["val_cv_mse"] = ev["constant_velocity"]["mse"]
+        curve.append(rec)
+        log(f"  epoch {ep}: {rec}")
+    return prior, curve
+
+
+def save(prior: kf.Prior, path: str, meta: dict) -> None:
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    prior.get_params().astype("<f4").tofile(path)
+    with open(os.path.splitext(path)[0] + ".json", "w") as f:
+        json.dump(meta, f, indent=1)
+
+
+def load(prior: kf.Prior, path: str = DEFAULT_WEIGHTS) -> None:
+    p = np.fromfile(path, dtype="<f4")
+    prior.set_params(p)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--trajectories", type=int, default=200)
+    ap.add_argument("--val-trajectories", type=int, default=10)
+    ap.add_argument("--frames", type=int, default=101)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    args = ap.parse_args()
+
+    def log(m):
+        print(m, flush=True)
+
+    t0 = time.perf_counter()
+    log(f"generating {args.trajectories} training trajectories")
+    ds = generate(range(10000, 10000 + args.trajectories), args.frames, log=log)
+    log(f"generating {args.val_trajectories} validation trajectories")
+    val = generate(range(10500, 10500 + args.val_trajectories), args.frames, log=log)
+    tgen = time.perf_counter() - t0
+    log(f"samples: train {ds.n}, val {val.n} ({tgen:.0f}s)")
+    t1 = time.perf_counter()
+    prior, curve = train(ds, args.epochs, args.batch, args.lr, log=log, val=val)
+    ttrain = time.perf_counter() - t1
+    ev = evaluate(prior, val)
+    log(f"validation: {json.dumps(ev)}")
+    meta = {"train_samples": ds.n, "val_samples": val.n, "train_trajectory_seeds": [10000, 10000 + args.trajectories - 1],
+            "val_trajectory_seeds": [10500, 10500 + args.val_trajectories - 1], "frames": args.frames,
+            "max_trans_m": 0.06, "max_rot_deg": 4.0, "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
+            "optimizer": "adam(0.9, 0.999, 1e-8)", "data_seconds": round(tgen, 1), "train_seconds": round(ttrain, 1),
+            "curve": curve, "validation": ev, "param_count": prior.nparams,
+            "label_source": "plain ICP (constant-velocity guess, 10/5/4) on the FP16 512^3 pipeline"}
+    save(prior, args.out, meta)
+    log(f"wrote {args.out}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

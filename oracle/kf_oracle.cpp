@@ -165,6 +165,23 @@ void mat4_mul(const double* A, const double* B, double* C) {
                      A[i * 4 + 3] * B[3 * 4 + j];
 }
 
+// One Newton step of the polar decomposition, R <- R (3I - R^T R) / 2, applied to
+// the constant-velocity prediction (whose product T1 T0^-1 T1 would amplify any
+// drift from orthonormality frame after frame).
+void orthonormalize(double* T) {
+  double S[9], R[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      S[i * 3 + j] = (T[0 * 4 + i] * T[0 * 4 + j] + T[1 * 4 + i] * T[1 * 4 + j]) + T[2 * 4 + i] * T[2 * 4 + j];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      const double rs = (T[r * 4 + 0] * S[0 * 3 + c] + T[r * 4 + 1] * S[1 * 3 + c]) + T[r * 4 + 2] * S[2 * 3 + c];
+      R[r * 3 + c] = 1.5 * T[r * 4 + c] - 0.5 * rs;
+    }
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) T[r * 4 + c] = R[r * 3 + c];
+}
+
 // Working-precision poses from T_wc: FP32 roundings for float, exact for double.
 template <typename R>
 void poses_from(const double* Twc, Pose<R>& wc, Pose<R>& cw) {
@@ -842,6 +859,7 @@ struct Pipeline : PipelineBase {
       rigid_inverse(Tprev2, Ti);
       mat4_mul(Ti, Tprev1, Dl);
       mat4_mul(Tprev1, Dl, G);
+      orthonormalize(G);
       icp_track<R>(d.intr, d.icp, live, model, G, Tcur, &stats);
     }
     auto t2 = now();
@@ -991,6 +1009,7 @@ void kfo_constant_velocity(const double* T2, const double* T1, double* G) {
   rigid_inverse(T2, Ti);
   mat4_mul(Ti, T1, Dl);
   mat4_mul(T1, Dl, G);
+  orthonormalize(G);
 }
 
 // Pipelines: precision 0 = FP32 (volume format from the desc), 1 = FP64.

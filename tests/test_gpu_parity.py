@@ -95,7 +95,7 @@ def test_integrate_and_raycast_bit_exact(gpu, kf, orc, synth, small, fmt):
         for nm in PLANES:
             assert same_bits(model.plane(lvl, nm), om.plane(lvl, nm)), (lvl, nm)
     valid = ~np.isnan(model.plane(0, "vz"))
-    assert valid.mean() > 0.3
+    assert valid.mean() > 0.15  # the small room's walls lie on the volume faces: only primitives render
 
 
 def test_icp_reduce_and_track_parity(gpu, kf, orc, synth, small):

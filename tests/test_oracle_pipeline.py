@@ -40,3 +40,17 @@ def test_fp64_oracle_agrees_with_fp32(orc, synth, small):
     for a, b in zip(e32, e64):
         assert metrics.translation_error(a, b) < 1e-3
         assert metrics.rotation_error_deg(a, b) < 0.05
+
+
+def test_poses_stay_orthonormal(orc, synth, small):
+    est, _ = _run(orc, synth, small, 0)
+    for T in est:
+        assert np.max(np.abs(T[:3, :3] @ T[:3, :3].T - np.eye(3))) < 1e-13
+
+
+def test_rotation_error_metric_small_angles():
+    from tests.helpers import pose
+    for ang in (1e-7, 1e-5, 1e-3, 0.1, 1.0):
+        a = pose(0.3, 0.2, 0.1)
+        b = a @ pose(0.0, 0.0, ang)
+        assert abs(metrics.rotation_error_deg(a, b) - np.degrees(ang)) < 1e-9 + 1e-9 * np.degrees(ang)
