@@ -112,41 +112,86 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
   }
   f3 vert = f3{nanf_(), nanf_(), nanf_()}, nrm = vert;
   int nsteps = 0;
-  if (active && hit && tnear < tfar) {
-    float t = tnear, fp;
-    f3 p = f3{o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z};
-    bool ok = sample<FMT>(a, p, fp) && fp >= 0.0f;
-    for (int s = 0; ok && s < a.max_steps; ++s) {
-      ++nsteps;
-      const float st = (fp >= 1.0f) ? a.step_far_m : a.step_near_m;
-      const float tn = t + st;
-      if (tn > tfar) break;
-      p = f3{o.x + tn * dir.x, o.y + tn * dir.y, o.z + tn * dir.z};
-      float fc;
-      if (!sample<FMT>(a, p, fc)) break;
-      if (fc < 0.0f) {
-        const float ts = t + st * (fp / (fp - fc));
-        const f3 ph = f3{o.x + ts * dir.x, o.y + ts * dir.y, o.z + ts * dir.z};
-        float gxp, gxm, gyp, gym, gzp, gzm;
-        const bool g = sample<FMT>(a, f3{ph.x + a.h, ph.y, ph.z}, gxp) &&
-                       sample<FMT>(a, f3{ph.x - a.h, ph.y, ph.z}, gxm) &&
-                       sample<FMT>(a, f3{ph.x, ph.y + a.h, ph.z}, gyp) &&
-                       sample<FMT>(a, f3{ph.x, ph.y - a.h, ph.z}, gym) &&
-                       sample<FMT>(a, f3{ph.x, ph.y, ph.z + a.h}, gzp) &&
-                       sample<FMT>(a, f3{ph.x, ph.y, ph.z - a.h}, gzm);
-        if (g) {
-          const f3 gr = f3{gxp - gxm, gyp - gym, gzp - gzm};
-          const float gl = sqrtf(dot(gr, gr));
-          if (gl > 0.0f) {
-            const float ig = 1.0f / gl;
-            nrm = f3{gr.x * ig, gr.y * ig, gr.z * ig};
-            vert = ph;
-          }
-        }
-        break;
+  // Zero crossing between t (sample fp >= 0) and t + st (sample fc < 0): linear
+  // refinement and trilinear central-difference normal.
+  auto hit_at = [&](float t, float st, float fp, float fc) {
+    const float ts = t + st * (fp / (fp - fc));
+    const f3 ph = f3{o.x + ts * dir.x, o.y + ts * dir.y, o.z + ts * dir.z};
+    float gxp, gxm, gyp, gym, gzp, gzm;
+    const bool g = sample<FMT>(a, f3{ph.x + a.h, ph.y, ph.z}, gxp) &&
+                   sample<FMT>(a, f3{ph.x - a.h, ph.y, ph.z}, gxm) &&
+                   sample<FMT>(a, f3{ph.x, ph.y + a.h, ph.z}, gyp) &&
+                   sample<FMT>(a, f3{ph.x, ph.y - a.h, ph.z}, gym) &&
+                   sample<FMT>(a, f3{ph.x, ph.y, ph.z + a.h}, gzp) &&
+                   sample<FMT>(a, f3{ph.x, ph.y, ph.z - a.h}, gzm);
+    if (g) {
+      const f3 gr = f3{gxp - gxm, gyp - gym, gzp - gzm};
+      const float gl = sqrtf(dot(gr, gr));
+      if (gl > 0.0f) {
+        const float ig = 1.0f / gl;
+        nrm = f3{gr.x * ig, gr.y * ig, gr.z * ig};
+        vert = ph;
       }
-      t = tn;
-      fp = fc;
+    }
+  };
+  if (active && hit && tnear < tfar) {
+    // The march: step_far while the last sample is +1 (free space), step_near
+    // otherwise; stop at the first sample < 0, outside the domain, past tfar or
+    // after max_steps.  In free space kSpec consecutive samples are fetched
+    // together (their positions are the sequential ones, t += step_far) and then
+    // consumed in order, so the result and the step count equal the sequential
+    // march while the dependent-load chain is kSpec times shorter.
+    constexpr int kSpec = 4;
+    float t = tnear, fp;
+    bool ok = sample<FMT>(a, f3{o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z}, fp) && fp >= 0.0f;
+    int s = 0;
+    while (ok && s < a.max_steps) {
+      if (fp >= 1.0f) {
+        float tk[kSpec], fk[kSpec];
+        bool vk[kSpec];
+        float tt = t;
+#pragma unroll
+        for (int j = 0; j < kSpec; ++j) {
+          tt = tt + a.step_far_m;
+          tk[j] = tt;
+          fk[j] = 0.0f;
+          vk[j] = tk[j] <= tfar &&
+                  sample<FMT>(a, f3{o.x + tt * dir.x, o.y + tt * dir.y, o.z + tt * dir.z}, fk[j]);
+        }
+        bool done = false;
+#pragma unroll
+        for (int j = 0; j < kSpec; ++j) {
+          if (s >= a.max_steps || !(tk[j] <= tfar) || !vk[j]) {
+            if (s < a.max_steps) ++nsteps;  // this step was taken and ended the march
+            done = true;
+            break;
+          }
+          ++nsteps;
+          ++s;
+          if (fk[j] < 0.0f) {
+            hit_at(t, a.step_far_m, fp, fk[j]);
+            done = true;
+            break;
+          }
+          t = tk[j];
+          fp = fk[j];
+          if (fp < 1.0f) break;  // the next step is a near step
+        }
+        if (done) break;
+      } else {
+        ++nsteps;
+        ++s;
+        const float tn = t + a.step_near_m;
+        if (tn > tfar) break;
+        float fc;
+        if (!sample<FMT>(a, f3{o.x + tn * dir.x, o.y + tn * dir.y, o.z + tn * dir.z}, fc)) break;
+        if (fc < 0.0f) {
+          hit_at(t, a.step_near_m, fp, fc);
+          break;
+        }
+        t = tn;
+        fp = fc;
+      }
     }
   }
   if (active) {

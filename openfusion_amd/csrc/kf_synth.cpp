@@ -92,66 +92,62 @@ double rel_trans(const double* Ta, const double* Tb) {
   return norm(d);
 }
 
-// Smooth random path: sums of sinusoids for position and yaw/pitch/roll.
+// Smooth random path: a wobbling orbit around the room centre.  The parameter s
+// is the orbit angle; radius, height, look-at target and roll are sums of random
+// sinusoids of s, and the camera looks at the target near the centre, so the far
+// side of the room and its primitives stay in view (a camera staring at a bare wall
+// sees a plane, which leaves ICP's translation along it unobservable).
 struct Path {
-  V3 c;
-  double ax[3], ay[3], az[3], wx[3], wy[3], wz[3], px[3], py[3], pz[3];
-  double yaw0, yb[3], yw[3], yp[3];
-  double pb[2], pw[2], pp[2];
-  double rb[2], rw[2], rp[2];
+  double r0, dir, th0;
+  double ra[3], rw[3], rp[3];   // radius wobble
+  double ya[3], yw[3], yp[3];   // height
+  double ta[3][3], tw[3][3], tp[3][3];  // look-at target wobble (x, y, z)
+  double qa[2], qw[2], qp[2];   // roll
 
   void init(Rng& g, const double* room, double margin) {
-    const double hx = room[0] * 0.5 - margin, hy = room[1] * 0.5 - margin, hz = room[2] * 0.5 - margin;
-    c = {0.0, 0.0, 0.0};
-    const double fr[3] = {0.55, 0.3, 0.15};
+    const double half = 0.5 * std::min(room[0], room[2]);
+    r0 = std::max(0.3, half - margin - 0.1 * half);
+    dir = g.u() < 0.5 ? -1.0 : 1.0;
+    th0 = g.range(0, 2 * kPi);
+    const double hy = 0.5 * room[1];
     for (int i = 0; i < 3; ++i) {
-      ax[i] = hx * fr[i] * g.range(0.7, 1.0);
-      ay[i] = hy * fr[i] * g.range(0.3, 0.6);
-      az[i] = hz * fr[i] * g.range(0.7, 1.0);
-      wx[i] = g.range(0.6, 1.4) * (i + 1);
-      wy[i] = g.range(0.6, 1.4) * (i + 1);
-      wz[i] = g.range(0.6, 1.4) * (i + 1);
-      px[i] = g.range(0, 2 * kPi);
-      py[i] = g.range(0, 2 * kPi);
-      pz[i] = g.range(0, 2 * kPi);
-    }
-    yaw0 = g.range(0, 2 * kPi);
-    for (int i = 0; i < 3; ++i) {
-      yb[i] = g.range(0.4, 1.0) / (i + 1);
-      yw[i] = g.range(0.5, 1.5) * (i + 1);
+      ra[i] = 0.1 * r0 * g.range(0.3, 1.0) / (i + 1);
+      rw[i] = g.range(0.7, 1.5) * (i + 1);
+      rp[i] = g.range(0, 2 * kPi);
+      ya[i] = 0.25 * hy * g.range(0.3, 1.0) / (i + 1);
+      yw[i] = g.range(0.7, 1.5) * (i + 1);
       yp[i] = g.range(0, 2 * kPi);
+      for (int k = 0; k < 3; ++k) {
+        ta[k][i] = 0.15 * r0 * g.range(0.3, 1.0) / (i + 1);
+        tw[k][i] = g.range(0.7, 2.0) * (i + 1);
+        tp[k][i] = g.range(0, 2 * kPi);
+      }
     }
     for (int i = 0; i < 2; ++i) {
-      pb[i] = g.range(0.08, 0.2) / (i + 1);
-      pw[i] = g.range(0.7, 1.6) * (i + 1);
-      pp[i] = g.range(0, 2 * kPi);
-      rb[i] = g.range(0.02, 0.06) / (i + 1);
-      rw[i] = g.range(0.7, 1.6) * (i + 1);
-      rp[i] = g.range(0, 2 * kPi);
+      qa[i] = g.range(0.02, 0.06) / (i + 1);
+      qw[i] = g.range(0.7, 1.6) * (i + 1);
+      qp[i] = g.range(0, 2 * kPi);
     }
   }
   void eval(double s, double* T) const {
-    V3 p = c;
+    double r = r0, y = 0.0, roll = 0.0;
+    double tg[3] = {0.0, 0.0, 0.0};
     for (int i = 0; i < 3; ++i) {
-      p.x += ax[i] * sin(wx[i] * s + px[i]);
-      p.y += ay[i] * sin(wy[i] * s + py[i]);
-      p.z += az[i] * sin(wz[i] * s + pz[i]);
+      r += ra[i] * sin(rw[i] * s + rp[i]);
+      y += ya[i] * sin(yw[i] * s + yp[i]);
+      for (int k = 0; k < 3; ++k) tg[k] += ta[k][i] * sin(tw[k][i] * s + tp[k][i]);
     }
-    double yaw = yaw0, pitch = 0.0, roll = 0.0;
-    for (int i = 0; i < 3; ++i) yaw += yb[i] * sin(yw[i] * s + yp[i]) + 0.35 * s;
-    for (int i = 0; i < 2; ++i) {
-      pitch += pb[i] * sin(pw[i] * s + pp[i]);
-      roll += rb[i] * sin(rw[i] * s + rp[i]);
-    }
-    // Base camera: forward +z world, down -y world, right = down x forward.
-    const double R0[9] = {-1, 0, 0, 0, -1, 0, 0, 0, 1};
-    double Ry[9], Rx[9], Rz[9], A[9], B[9], R[9];
-    rot_y(yaw, Ry);
-    rot_x(pitch, Rx);
-    rot_z(roll, Rz);
-    mat3_mul(Ry, Rx, A);
-    mat3_mul(A, R0, B);
-    mat3_mul(B, Rz, R);
+    for (int i = 0; i < 2; ++i) roll += qa[i] * sin(qw[i] * s + qp[i]);
+    const double th = th0 + dir * s;
+    const V3 p = {r * sin(th), y, r * cos(th)};
+    const V3 fwd = normalize(V3{tg[0], tg[1], tg[2]} - p);
+    const V3 right0 = normalize(cross(V3{0, -1, 0}, fwd));
+    const V3 down0 = cross(fwd, right0);
+    // roll about the viewing direction
+    const double cr = cos(roll), sr = sin(roll);
+    const V3 right = right0 * cr + down0 * sr;
+    const V3 down = down0 * cr - right0 * sr;
+    const double R[9] = {right.x, down.x, fwd.x, right.y, down.y, fwd.y, right.z, down.z, fwd.z};
     set_pose(T, R, p);
   }
 };

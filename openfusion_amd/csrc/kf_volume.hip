@@ -155,7 +155,8 @@ __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
         if (upd & (1u << j)) {
           const float fo = half_bits_to_float(w[j] & 0xffffu);
           const float wo = (float)(w[j] >> 16);
-          const float fn = (fo * wo + fs[j]) / (wo + 1.0f);
+          // Free space (f_old = f = 1) averages to exactly 1: (w + 1) / (w + 1).
+          const float fn = (fo == 1.0f && fs[j] == 1.0f) ? 1.0f : (fo * wo + fs[j]) / (wo + 1.0f);
           const float wn = fminf(wo + 1.0f, a.maxw);
           w[j] = float_to_half_bits(fn) | ((uint32_t)wn << 16);
         }
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
       for (int j = 0; j < 2; ++j) {
         if (upd & (1u << j)) {
           const float fo = fw[2 * j], wo = fw[2 * j + 1];
-          fw[2 * j] = (fo * wo + fs[j]) / (wo + 1.0f);
+          fw[2 * j] = (fo == 1.0f && fs[j] == 1.0f) ? 1.0f : (fo * wo + fs[j]) / (wo + 1.0f);
           fw[2 * j + 1] = fminf(wo + 1.0f, a.maxw);
         }
       }

@@ -100,8 +100,8 @@ class Preset:
     scene_seed: int = 1001
     traj_seed: int = 3001
     noise_seed: int = 2001
-    margin: float = 0.7
-    clearance: float = 0.45
+    margin: float = 0.4
+    clearance: float = 0.35
     max_weight: float = 128.0
     min_depth: float = 0.3
     max_depth: float = 4.0
@@ -115,7 +115,7 @@ PRESETS = {
     "full16": Preset("full16", 512, 4.0, 0.02, kf.KF_VOXEL_F16, 300, (3.0, 3.0, 3.0), n_random=(6, 10)),
     "stress": Preset("stress", 768, 6.0, 0.03, kf.KF_VOXEL_F16, 1000, (6.0, 3.0, 6.0), n_random=(20, 20),
                      traj=KF_TRAJ_BURST, max_trans=0.06, max_rot=5.0, scene_seed=1004, traj_seed=3004,
-                     noise_seed=2004, margin=1.0),
+                     noise_seed=2004, margin=0.8),
 }
 
 
