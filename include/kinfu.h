@@ -151,6 +151,18 @@ kf_status kf_volume_data(const kf_volume* vol, void** d_voxels, size_t* bytes);
  * observed voxels, voxels with f < 0 and w > 0}. */
 kf_status kf_volume_stats(const kf_volume* vol, double* d_out, kf_stream s);
 
+/* Raw tsdf export (replaces RunApp::saveVolumeToFile, Source/src/MainApp.cpp:181-275):
+ * d_out (device, n^3 floats) = tsdf in metres in the reference's x*n^2 + y*n + z
+ * order; unobserved voxels read +T. */
+kf_status kf_volume_export_tsdf(const kf_volume* vol, float* d_out, kf_stream s);
+/* Surface extraction by marching tetrahedra (the reference has only a disabled
+ * marching-cubes sketch, Source/src/MainApp.cpp:277-328).  Writes up to max_tris
+ * triangles (9 floats each: three world-space vertices, facing free space) to
+ * d_tris and the number found (may exceed max_tris) to *d_count (device uint32).
+ * Triangle order is not deterministic. */
+kf_status kf_volume_extract_mesh(const kf_volume* vol, float* d_tris, uint32_t max_tris, uint32_t* d_count,
+                                 kf_stream s);
+
 /* ------------------------------------------------------------------------ */
 /* Depth preprocessing: uint16 mm depth -> raylen (L0), bilateral-filtered depth,
  * 3-level pyramid, camera-frame vertex and normal maps per level.  Replaces
@@ -168,6 +180,16 @@ typedef struct kf_preprocess_params {
 
 kf_status kf_preprocess(const kf_intrinsics* k0, const kf_preprocess_params* p,
                         const uint16_t* d_depth, kf_frame_maps* live, kf_stream s);
+
+/* Camera-side depth conditioning (optional, ahead of kf_preprocess; in != out).
+ * kf_depth_median replaces d_MedianFilter (Source/src/CameraSupport/CameraAbstract.cu:100-124):
+ * 5x5 median of the non-null neighbours, null if at most 12 are valid.
+ * kf_depth_undistort replaces d_undistort (CameraAbstract.cu:160-221): Brown-Conrady
+ * remap with dist = {k1, k2, p1, p2}, bilinear, null unless all 4 sources are valid. */
+kf_status kf_depth_median(const uint16_t* d_in, uint16_t* d_out, int32_t width, int32_t height,
+                          uint16_t null_value, kf_stream s);
+kf_status kf_depth_undistort(const uint16_t* d_in, uint16_t* d_out, const kf_intrinsics* k, const float dist[4],
+                             kf_stream s);
 
 /* ------------------------------------------------------------------------ */
 /* TSDF integration sweep.  Replaces VolumeIntegrator::Integrate / d_integrateVolume
@@ -202,6 +224,12 @@ typedef struct kf_raycast_params {
 kf_status kf_raycast(const kf_volume* vol, const kf_intrinsics* k0, const kf_raycast_params* p,
                      const kf_pose* d_pose, kf_frame_maps* model,
                      unsigned long long* d_steps, kf_stream s);
+
+/* Shaded view of the model maps (the reference's render output,
+ * Source/src/Raycaster.cu:214-228): mode 0 = grey 0.6 * dot(-ray, n) + 0.2,
+ * mode 1 = normal map (n + 1) / 2; RGBA8 (alpha 255), 0 where the model is invalid. */
+kf_status kf_render_shaded(const kf_intrinsics* k0, const kf_frame_maps* model, const kf_pose* d_pose,
+                           int32_t mode, uint32_t* d_rgba, kf_stream s);
 
 /* ------------------------------------------------------------------------ */
 /* ICP tracker.  Replaces ICP::Iterate_ICP, ICP::FindCorresponding and

@@ -1,7 +1,8 @@
 """Builds the native libraries in-tree.
 
 * ``openfusion_amd/libkinfu.so``  — HIP kernels + C ABI (hipcc, gfx950)
-* ``openfusion_amd/libkfsynth.so`` — synthetic depth-stream generator (g++, OpenMP)
+* ``openfusion_amd/libkfsynth.so`` — host frame sources: synthetic streams and the
+  recorded-stream format (g++, OpenMP)
 * ``oracle/libkforacle.so``        — the CPU oracle (g++, OpenMP), via oracle/Makefile
 
 Every call recompiles.  Usage: ``python -m openfusion_amd.build [--no-oracle]``.
@@ -38,6 +39,8 @@ LIB_SOURCES = [
     "kf_raycast.hip",
     "kf_icp.hip",
     "kf_prior.hip",
+    "kf_export.hip",
+    "kf_conditioning.hip",
 ]
 
 
@@ -74,7 +77,7 @@ def build_synth() -> str:
     out = os.path.join(PKG, "libkfsynth.so")
     tmp = out + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
-          f"-I{INCLUDE}", "-o", tmp, os.path.join(CSRC, "kf_synth.cpp")])
+          f"-I{INCLUDE}", "-o", tmp, os.path.join(CSRC, "kf_synth.cpp"), os.path.join(CSRC, "kf_recording.cpp")])
     os.replace(tmp, out)
     return out
 

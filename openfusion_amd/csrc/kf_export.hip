@@ -1,0 +1,198 @@
+// kf_export.hip — volume export, surface extraction and shaded rendering.
+//
+//  * tsdf_export: the reference's raw volume dump (RunApp::saveVolumeToFile,
+//    Source/src/MainApp.cpp:181-275 — n^3 float32 tsdf in x*n^2 + y*n + z order)
+//    with the tsdf in metres (f * T) and unobserved voxels (w = 0) at +T.
+//  * mesh_tetra: surface extraction by marching tetrahedra (each cell split into
+//    the six Freudenthal tetrahedra around its main diagonal, so neighbouring cells
+//    share faces and the mesh is watertight); the reference only has a disabled
+//    marching-cubes sketch (MainApp.cpp:277-328).  Cells with an unobserved corner
+//    are skipped.  Triangles face the positive (free-space) side.
+//  * render_shaded: Lambert shading 0.6 * dot(-ray, n) + 0.2 of the ray-cast model
+//    (Source/src/Raycaster.cu:214-228) or the normal map (n + 1) / 2, as RGBA8.
+#include "kf_device.hpp"
+#include "kf_internal.hpp"
+
+namespace kf {
+namespace {
+
+template <int FMT>
+__device__ __forceinline__ void vox_fw(const void* vox, size_t i, float& f, float& w) {
+  if (FMT == KF_VOXEL_F16) {
+    const uint32_t q = static_cast<const uint32_t*>(vox)[i];
+    f = half_bits_to_float(q & 0xffffu);
+    w = (float)(q >> 16);
+  } else {
+    const float2 q = static_cast<const float2*>(vox)[i];
+    f = q.x;
+    w = q.y;
+  }
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void tsdf_export(const void* vox, size_t n3, float trunc, float* out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n3) return;
+  float f, w;
+  vox_fw<FMT>(vox, i, f, w);
+  out[i] = w > 0.0f ? f * trunc : trunc;
+}
+
+// Freudenthal split: corner i = dx + 2 dy + 4 dz; tetrahedra (0, 7, a, b) for the
+// cycle 1 -> 3 -> 2 -> 6 -> 4 -> 5 -> 1 around the diagonal 0-7.
+__constant__ int kTet[6][4] = {{0, 7, 1, 3}, {0, 7, 3, 2}, {0, 7, 2, 6}, {0, 7, 6, 4}, {0, 7, 4, 5}, {0, 7, 5, 1}};
+
+__device__ __forceinline__ f3 edge_point(f3 pa, f3 pb, float fa, float fb) {
+  const float t = fa / (fa - fb);
+  return f3{pa.x + t * (pb.x - pa.x), pa.y + t * (pb.y - pa.y), pa.z + t * (pb.z - pa.z)};
+}
+
+__device__ __forceinline__ void emit(f3 a, f3 b, f3 c, f3 g, float* tris, unsigned* count, unsigned cap) {
+  const f3 nt = cross(sub(b, a), sub(c, a));
+  if (dot(nt, g) < 0.0f) {
+    const f3 t = b;
+    b = c;
+    c = t;
+  }
+  const unsigned k = atomicAdd(count, 1u);
+  if (k >= cap) return;
+  float* o = tris + (size_t)k * 9;
+  o[0] = a.x; o[1] = a.y; o[2] = a.z;
+  o[3] = b.x; o[4] = b.y; o[5] = b.z;
+  o[6] = c.x; o[7] = c.y; o[8] = c.z;
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void mesh_tetra(const void* vox, int n, float h, float ox, float oy, float oz,
+                                                  float* tris, unsigned* count, unsigned cap) {
+  const long cells = (long)(n - 1) * (n - 1) * (n - 1);
+  const long ci = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= cells) return;
+  const int z = (int)(ci % (n - 1));
+  const int y = (int)((ci / (n - 1)) % (n - 1));
+  const int x = (int)(ci / ((long)(n - 1) * (n - 1)));
+  float f[8];
+  f3 p[8];
+  int neg = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+    float w;
+    vox_fw<FMT>(vox, ((size_t)(x + dx) * n + (y + dy)) * n + (z + dz), f[c], w);
+    if (!(w > 0.0f)) return;
+    neg += f[c] < 0.0f ? 1 : 0;
+    p[c] = f3{ox + ((float)(x + dx) + 0.5f) * h, oy + ((float)(y + dy) + 0.5f) * h, oz + ((float)(z + dz) + 0.5f) * h};
+  }
+  if (neg == 0 || neg == 8) return;
+  for (int t = 0; t < 6; ++t) {
+    int v[4], nn = 0, pn = 0, ni[4], pi[4];
+    for (int j = 0; j < 4; ++j) {
+      v[j] = kTet[t][j];
+      if (f[v[j]] < 0.0f) ni[nn++] = v[j]; else pi[pn++] = v[j];
+    }
+    if (nn == 0 || nn == 4) continue;
+    // gradient direction: from the negative corners towards the positive ones
+    f3 cn = f3{0, 0, 0}, cp = f3{0, 0, 0};
+    for (int j = 0; j < nn; ++j) cn = f3{cn.x + p[ni[j]].x, cn.y + p[ni[j]].y, cn.z + p[ni[j]].z};
+    for (int j = 0; j < pn; ++j) cp = f3{cp.x + p[pi[j]].x, cp.y + p[pi[j]].y, cp.z + p[pi[j]].z};
+    const float in = 1.0f / (float)nn, ip = 1.0f / (float)pn;
+    const f3 g = f3{cp.x * ip - cn.x * in, cp.y * ip - cn.y * in, cp.z * ip - cn.z * in};
+    if (nn == 1 || nn == 3) {
+      const int odd = nn == 1 ? ni[0] : pi[0];
+      const int* oth = nn == 1 ? pi : ni;
+      const f3 a = edge_point(p[odd], p[oth[0]], f[odd], f[oth[0]]);
+      const f3 b = edge_point(p[odd], p[oth[1]], f[odd], f[oth[1]]);
+      const f3 c = edge_point(p[odd], p[oth[2]], f[odd], f[oth[2]]);
+      emit(a, b, c, g, tris, count, cap);
+    } else {
+      const f3 a = edge_point(p[ni[0]], p[pi[0]], f[ni[0]], f[pi[0]]);
+      const f3 b = edge_point(p[ni[0]], p[pi[1]], f[ni[0]], f[pi[1]]);
+      const f3 c = edge_point(p[ni[1]], p[pi[1]], f[ni[1]], f[pi[1]]);
+      const f3 d = edge_point(p[ni[1]], p[pi[0]], f[ni[1]], f[pi[0]]);
+      emit(a, b, c, g, tris, count, cap);
+      emit(a, c, d, g, tris, count, cap);
+    }
+  }
+}
+
+__global__ void zero_u32(uint32_t* p) {
+  if (threadIdx.x == 0) *p = 0u;
+}
+
+__device__ __forceinline__ uint32_t rgba8(float r, float g, float b, float a) {
+  r = fminf(fmaxf(r, 0.0f), 1.0f);
+  g = fminf(fmaxf(g, 0.0f), 1.0f);
+  b = fminf(fmaxf(b, 0.0f), 1.0f);
+  a = fminf(fmaxf(a, 0.0f), 1.0f);
+  return ((uint32_t)(a * 255.0f) << 24) | ((uint32_t)(b * 255.0f) << 16) | ((uint32_t)(g * 255.0f) << 8) |
+         (uint32_t)(r * 255.0f);
+}
+
+__global__ __launch_bounds__(256) void render_shaded(kf_level_maps m, const kf_pose* pose, float fx, float fy,
+                                                     float cx, float cy, int mode, uint32_t* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.width * m.height) return;
+  const int y = i / m.width, x = i - y * m.width;
+  const f3 n = f3{m.nx[i], m.ny[i], m.nz[i]};
+  if (!finite3(n)) {
+    out[i] = 0u;
+    return;
+  }
+  if (mode == 1) {
+    out[i] = rgba8((n.x + 1.0f) * 0.5f, (n.y + 1.0f) * 0.5f, (n.z + 1.0f) * 0.5f, 1.0f);
+    return;
+  }
+  const kf_posef P = pose->wc;
+  const f3 dw = rot(P.r, f3{((float)x - cx) / fx, ((float)y - cy) / fy, 1.0f});
+  const float il = 1.0f / sqrtf(dot(dw, dw));
+  const float d = -(dw.x * il * n.x + dw.y * il * n.y + dw.z * il * n.z);
+  const float g = fabsf(d) < 1e-6f ? 0.0f : 0.6f * d + 0.2f;
+  out[i] = rgba8(g, g, g, 1.0f);
+}
+
+}  // namespace
+}  // namespace kf
+
+using namespace kf;
+
+extern "C" kf_status kf_volume_export_tsdf(const kf_volume* v, float* d_out, kf_stream s) {
+  KF_CHECK_ARG(v && d_out, "NULL argument");
+  const size_t n3 = (size_t)v->desc.n * v->desc.n * v->desc.n;
+  const unsigned blocks = (unsigned)((n3 + 255) / 256);
+  if (v->desc.format == KF_VOXEL_F16)
+    hipLaunchKernelGGL(tsdf_export<KF_VOXEL_F16>, dim3(blocks), dim3(256), 0, S(s), (const void*)v->d_voxels, n3,
+                       v->desc.trunc_m, d_out);
+  else
+    hipLaunchKernelGGL(tsdf_export<KF_VOXEL_F32>, dim3(blocks), dim3(256), 0, S(s), (const void*)v->d_voxels, n3,
+                       v->desc.trunc_m, d_out);
+  return check_launch("tsdf_export");
+}
+
+extern "C" kf_status kf_volume_extract_mesh(const kf_volume* v, float* d_tris, uint32_t max_tris,
+                                            uint32_t* d_count, kf_stream s) {
+  KF_CHECK_ARG(v && d_count && (d_tris || max_tris == 0), "NULL argument");
+  const kf_volume_desc& d = v->desc;
+  const long cells = (long)(d.n - 1) * (d.n - 1) * (d.n - 1);
+  const float h = d.extent_m / (float)d.n;
+  hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, S(s), d_count);  // a kernel, so graph capture is safe
+  const unsigned blocks = (unsigned)((cells + 255) / 256);
+  if (d.format == KF_VOXEL_F16)
+    hipLaunchKernelGGL(mesh_tetra<KF_VOXEL_F16>, dim3(blocks), dim3(256), 0, S(s), (const void*)v->d_voxels, d.n,
+                       h, d.origin_m[0], d.origin_m[1], d.origin_m[2], d_tris, d_count, max_tris);
+  else
+    hipLaunchKernelGGL(mesh_tetra<KF_VOXEL_F32>, dim3(blocks), dim3(256), 0, S(s), (const void*)v->d_voxels, d.n,
+                       h, d.origin_m[0], d.origin_m[1], d.origin_m[2], d_tris, d_count, max_tris);
+  return check_launch("mesh_tetra");
+}
+
+extern "C" kf_status kf_render_shaded(const kf_intrinsics* k0, const kf_frame_maps* model, const kf_pose* d_pose,
+                                      int32_t mode, uint32_t* d_rgba, kf_stream s) {
+  KF_CHECK_ARG(k0 && model && d_pose && d_rgba, "NULL argument");
+  KF_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (shaded) or 1 (normals)");
+  const kf_level_maps& m = model->level[0];
+  KF_CHECK_ARG(m.width == k0->width && m.height == k0->height && m.nx && m.ny && m.nz, "bad model maps");
+  const int P = m.width * m.height;
+  hipLaunchKernelGGL(render_shaded, dim3((P + 255) / 256), dim3(256), 0, S(s), m, d_pose, k0->fx, k0->fy, k0->cx,
+                     k0->cy, mode, d_rgba);
+  return check_launch("render_shaded");
+}

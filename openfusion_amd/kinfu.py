@@ -142,6 +142,11 @@ _PROTOS = {
     "kf_volume_get_desc": (C.c_int, [_VP, C.POINTER(VolumeDesc)]),
     "kf_volume_data": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(C.c_size_t)]),
     "kf_volume_stats": (C.c_int, [_VP, _VP, _VP]),
+    "kf_volume_export_tsdf": (C.c_int, [_VP, _VP, _VP]),
+    "kf_volume_extract_mesh": (C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP]),
+    "kf_depth_median": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, C.c_uint16, _VP]),
+    "kf_depth_undistort": (C.c_int, [_VP, _VP, C.POINTER(Intrinsics), C.POINTER(C.c_float), _VP]),
+    "kf_render_shaded": (C.c_int, [C.POINTER(Intrinsics), C.POINTER(FrameMaps), _VP, C.c_int32, _VP, _VP]),
     "kf_preprocess": (C.c_int, [C.POINTER(Intrinsics), C.POINTER(PreprocessParams), _VP,
                                 C.POINTER(FrameMaps), _VP]),
     "kf_integrate": (C.c_int, [_VP, C.POINTER(Intrinsics), C.POINTER(IntegrateParams), _VP, _VP, _VP, _VP,
@@ -446,6 +451,23 @@ class Volume:
         check(lib().kf_volume_stats(self.handle, out.ptr, None), "kf_volume_stats")
         return out.download(np.float64, 4)
 
+    def export_tsdf(self) -> np.ndarray:
+        """(n, n, n) float32 tsdf in metres, index [x, y, z]; unobserved voxels = +T."""
+        n = self.desc.n
+        out = DeviceBuffer(n ** 3 * 4)
+        check(lib().kf_volume_export_tsdf(self.handle, out.ptr, None), "kf_volume_export_tsdf")
+        return out.download(np.float32).reshape(n, n, n)
+
+    def extract_mesh(self, max_tris: int = 4_000_000) -> np.ndarray:
+        """(T, 3, 3) float32 triangles (world metres) from marching tetrahedra."""
+        tris = DeviceBuffer(max_tris * 36)
+        cnt = DeviceBuffer(4)
+        check(lib().kf_volume_extract_mesh(self.handle, tris.ptr, max_tris, cnt.ptr, None), "extract_mesh")
+        n = int(cnt.download(np.uint32)[0])
+        if n > max_tris:
+            return self.extract_mesh(n)
+        return tris.download(np.float32, n * 9).reshape(n, 3, 3)
+
     def tsdf_weight(self) -> tuple[np.ndarray, np.ndarray]:
         """(tsdf/T, weight) as float32 arrays of shape (n, n, n), index [x, y, z]."""
         n = self.desc.n
@@ -710,6 +732,32 @@ class Pipeline:
         ms = (C.c_float * STAGES)()
         check(lib().kf_pipeline_get_stage_ms(self.handle, ms), "get_stage_ms")
         return np.array(ms[:], dtype=np.float64)
+
+
+def depth_median(depth: np.ndarray, null_value: int = 0) -> np.ndarray:
+    """5x5 median of the valid neighbours (GPU), uint16 (H, W) -> (H, W)."""
+    H, W = depth.shape
+    din = upload(np.ascontiguousarray(depth, np.uint16))
+    dout = DeviceBuffer(din.nbytes)
+    check(lib().kf_depth_median(din.ptr, dout.ptr, W, H, null_value, None), "kf_depth_median")
+    return dout.download(np.uint16).reshape(H, W)
+
+
+def depth_undistort(depth: np.ndarray, k: Intrinsics, dist) -> np.ndarray:
+    """Brown-Conrady remap (GPU) with dist = (k1, k2, p1, p2)."""
+    H, W = depth.shape
+    din = upload(np.ascontiguousarray(depth, np.uint16))
+    dout = DeviceBuffer(din.nbytes)
+    d4 = (C.c_float * 4)(*[float(v) for v in dist])
+    check(lib().kf_depth_undistort(din.ptr, dout.ptr, C.byref(k), d4, None), "kf_depth_undistort")
+    return dout.download(np.uint16).reshape(H, W)
+
+
+def render_shaded(k0: Intrinsics, model: DeviceMaps, pose: DevicePose, mode: int = 0) -> np.ndarray:
+    """(H, W) uint32 RGBA8 view of the model maps (0 = shaded, 1 = normal map)."""
+    out = DeviceBuffer(k0.width * k0.height * 4)
+    check(lib().kf_render_shaded(C.byref(k0), C.byref(model.maps), pose.ptr, mode, out.ptr, None), "render_shaded")
+    return out.download(np.uint32).reshape(k0.height, k0.width)
 
 
 def exported_symbols() -> list[str]:

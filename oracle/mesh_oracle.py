@@ -1,0 +1,87 @@
+"""Numpy marching tetrahedra and shading restatements.  TEST INFRASTRUCTURE ONLY.
+
+Independent restatement of the surface extraction in the product's
+kf_volume_extract_mesh (six Freudenthal tetrahedra per cell around the 0-7
+diagonal, linear edge interpolation, cells with an unobserved corner skipped,
+triangles facing the positive side) and of the Lambert shading of
+kf_render_shaded, in float64.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+TETS = [(0, 7, 1, 3), (0, 7, 3, 2), (0, 7, 2, 6), (0, 7, 6, 4), (0, 7, 4, 5), (0, 7, 5, 1)]
+
+
+def marching_tetrahedra(f: np.ndarray, w: np.ndarray, h: float, origin) -> np.ndarray:
+    """f, w: (n, n, n) [x, y, z]; returns (T, 3, 3) triangles."""
+    n = f.shape[0]
+    corners = []
+    for c in range(8):
+        dx, dy, dz = c & 1, (c >> 1) & 1, (c >> 2) & 1
+        corners.append((dx, dy, dz))
+    F = np.stack([f[dx:n - 1 + dx, dy:n - 1 + dy, dz:n - 1 + dz] for dx, dy, dz in corners], -1).reshape(-1, 8)
+    Wc = np.stack([w[dx:n - 1 + dx, dy:n - 1 + dy, dz:n - 1 + dz] for dx, dy, dz in corners], -1).reshape(-1, 8)
+    idx = np.stack(np.meshgrid(np.arange(n - 1), np.arange(n - 1), np.arange(n - 1), indexing="ij"), -1).reshape(-1, 3)
+    neg = (F < 0).sum(1)
+    keep = np.all(Wc > 0, 1) & (neg > 0) & (neg < 8)
+    F, idx = F[keep].astype(np.float64), idx[keep]
+    off = np.array(corners, dtype=np.float64)
+    P = (idx[:, None, :] + off[None, :, :] + 0.5) * h + np.asarray(origin, np.float64)
+    out = []
+    for t in TETS:
+        ft = F[:, t]
+        pt = P[:, t]
+        for i in range(ft.shape[0]):
+            v = ft[i]
+            negs = [j for j in range(4) if v[j] < 0]
+            poss = [j for j in range(4) if v[j] >= 0]
+            if len(negs) in (0, 4):
+                continue
+            p = pt[i]
+            g = p[poss].mean(0) - p[negs].mean(0)
+
+            def e(a, b):
+                s = v[a] / (v[a] - v[b])
+                return p[a] + s * (p[b] - p[a])
+
+            if len(negs) in (1, 3):
+                odd = negs[0] if len(negs) == 1 else poss[0]
+                oth = poss if len(negs) == 1 else negs
+                tris = [(e(odd, oth[0]), e(odd, oth[1]), e(odd, oth[2]))]
+            else:
+                a, b = e(negs[0], poss[0]), e(negs[0], poss[1])
+                c, d = e(negs[1], poss[1]), e(negs[1], poss[0])
+                tris = [(a, b, c), (a, c, d)]
+            for A, B, Cc in tris:
+                if np.dot(np.cross(B - A, Cc - A), g) < 0:
+                    B, Cc = Cc, B
+                out.append((A, B, Cc))
+    return np.array(out, dtype=np.float64).reshape(-1, 3, 3)
+
+
+def shade(nx, ny, nz, T_wc, fx, fy, cx, cy, mode=0) -> np.ndarray:
+    """RGBA8 of the model normals (float32 arithmetic like the device)."""
+    H, W = nx.shape
+    f32 = np.float32
+    n = np.stack([nx, ny, nz], -1).astype(f32)
+    valid = np.all(np.isfinite(n), -1)
+    out = np.zeros((H, W), np.uint32)
+
+    def q(v):
+        return (np.clip(v, 0, 1) * f32(255.0)).astype(np.uint32)
+
+    if mode == 1:
+        r, g, b = [(n[..., k] + f32(1.0)) * f32(0.5) for k in range(3)]
+    else:
+        x, y = np.meshgrid(np.arange(W, dtype=f32), np.arange(H, dtype=f32))
+        R = T_wc[:3, :3].astype(f32)
+        xn, yn = (x - f32(cx)) / f32(fx), (y - f32(cy)) / f32(fy)
+        d = np.stack([(R[k, 0] * xn + R[k, 1] * yn) + R[k, 2] for k in range(3)], -1)
+        il = f32(1.0) / np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2])
+        dd = -((d[..., 0] * il * n[..., 0] + d[..., 1] * il * n[..., 1]) + d[..., 2] * il * n[..., 2])
+        gr = np.where(np.abs(dd) < f32(1e-6), f32(0.0), f32(0.6) * dd + f32(0.2))
+        r = g = b = gr
+    rgba = (q(np.ones_like(r)) << 24) | (q(b) << 16) | (q(g) << 8) | q(r)
+    out[valid] = rgba[valid]
+    return out
