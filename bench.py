@@ -189,6 +189,8 @@ def main() -> int:
     ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--only", choices=["plain", "fp16", "learned_fp16"], default=None)
     ap.add_argument("--no-prior", action="store_true", help="skip the learned-prior variant")
+    ap.add_argument("--workload", choices=["full", "stress"], default="full",
+                    help="full: 512^3 / 4 m, 300 frames (headline); stress: 768^3 / 6 m, 1000 frames with bursts")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
@@ -200,17 +202,21 @@ def main() -> int:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
     W, K = args.warmup, args.steps
+    stress = args.workload == "stress"
+    if stress and K == 290:
+        K = 990
     frames = W + K
+    P32, P16 = ("stress32", "stress") if stress else ("full", "full16")
     t = time.perf_counter()
-    stream = synth.make_stream("full", frames=max(frames, W + args.stage_frames, args.cpu_frames))
+    stream = synth.make_stream(P16 if stress else "full", frames=max(frames, W + args.stage_frames, args.cpu_frames))
     log(f"stream: {stream.depth.shape[0]} frames, {stream.scene.count} primitives ({time.perf_counter()-t:.1f}s)")
     dd = kf.upload(stream.depth)
 
     from openfusion_amd import prior_train
     have_prior = os.path.exists(prior_train.DEFAULT_WEIGHTS) and not args.no_prior
-    variants = [("plain", "full", False), ("fp16", "full16", False)]
+    variants = [("plain", P32, False), ("fp16", P16, False)]
     if have_prior:
-        variants.append(("learned_fp16", "full16", True))
+        variants.append(("learned_fp16", P16, True))
     res = {}
     for name, preset, use_prior in variants:
         if args.only and args.only != name:
@@ -220,7 +226,7 @@ def main() -> int:
         log(f"  {name}: {res[name]['fps']:.1f} fps, ATE {res[name]['ate_mm']:.3f} mm")
 
     head = "learned_fp16" if "learned_fp16" in res else ("fp16" if "fp16" in res else "plain")
-    head_preset = "full" if head == "plain" else "full16"
+    head_preset = P32 if head == "plain" else P16
     head_prior = head == "learned_fp16"
     log(f"stage timing ({head}, {args.stage_frames} frames)")
     sr = stage_run(kf, synth, stream, dd, head_preset, W, args.stage_frames, head_prior)
@@ -235,6 +241,7 @@ def main() -> int:
     ray_ms = sr["ms"][:, 4]
     S_bar = float(sr["steps"].mean() / P)
     ray_bytes = P * (S_bar * (2 if Bv == 4 else 4) + 24)
+    pre = synth.PRESETS[head_preset]
 
     out = {
         "metric": METRIC,
@@ -250,8 +257,9 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{n}^3 {'FP16+u16' if Bv == 4 else 'FP32+FP32'} TSDF over 4 m (T 0.02 m), 640x480, "
-                        f"seeded 300-frame synthetic stream, ICP 3 levels "
+            "workload": f"{n}^3 {'FP16+u16' if Bv == 4 else 'FP32+FP32'} TSDF over {pre.extent:g} m "
+                        f"(T {pre.trunc:g} m), 640x480, seeded {stream.depth.shape[0]}-frame synthetic stream "
+                        f"({'bursts up to 6 cm / 5 deg' if stress else '<= 3 cm / 2 deg'} per frame), ICP 3 levels "
                         + ("4/3/3 warm-started by the learned prior (fallback: plain 10/5/4)" if head_prior else
                            "10/5/4 from the constant-velocity guess"),
             "volume_format": "f16+u16" if Bv == 4 else "f32+f32",
@@ -291,10 +299,10 @@ def main() -> int:
     if not args.no_cpu:
         from oracle import oracle as orc
         log(f"cpu baseline: oracle FP32 pipeline, {args.cpu_frames} frames at 512^3")
-        cb = cpu_baseline(orc, synth, stream, "full", args.cpu_frames)
+        cb = cpu_baseline(orc, synth, stream, P32, args.cpu_frames)
         out["cpu_baseline"] = {
             "value": round(cb["fps"], 3), "unit": "fps", "cores": cb["threads"], "kind": "port",
-            "sample": f"frames 2..{args.cpu_frames - 1} of the 512^3 FP32 stream (median per-frame time of "
+            "sample": f"frames 2..{args.cpu_frames - 1} of the {n}^3 FP32 stream (median per-frame time of "
                       f"preprocess+ICP+integrate+raycast; frames 0-1 warm-up), {cpu_model()}",
             "stage_ms": [round(x, 2) for x in cb["stage_ms"]],
         }

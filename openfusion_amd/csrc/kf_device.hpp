@@ -76,6 +76,39 @@ __device__ __forceinline__ uint32_t float_to_half_bits(float f) {
   return (uint32_t)__builtin_bit_cast(uint16_t, v);
 }
 
+// DPP lane move (GFX9 encodings): quad_perm xor1 = 0xB1, xor2 = 0x4E; row_ror:4 =
+// 0x124, row_ror:8 = 0x128; row_bcast:15 = 0x142, row_bcast:31 = 0x143.  Lanes in
+// rows excluded by ROW_MASK receive 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK,
+                                                               0xf, false));
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, false);
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = dpp_i<CTRL, ROW_MASK>((int)(b & 0xffffffffll));
+  const int hi = dpp_i<CTRL, ROW_MASK>((int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the 64 lanes of a wavefront with DPP moves only (no LDS): quad xor
+// steps, row rotations, then the two row broadcasts.  The total is in lane 63.
+__device__ __forceinline__ int wave_sum_i_lane63(int v) {
+  v += dpp_i<0xB1>(v);
+  v += dpp_i<0x4E>(v);
+  v += dpp_i<0x124>(v);
+  v += dpp_i<0x128>(v);
+  v += dpp_i<0x142, 0xA>(v);
+  v += dpp_i<0x143, 0xC>(v);
+  return v;
+}
+
 // Wave-level sum of a float with the DPP/permute ladder, result valid in all lanes.
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll

@@ -14,8 +14,8 @@
 // of (A^T A) x = A^T b the rotation is -x[0..2] (ICP.cpp:162-168).
 //
 // Reduction: every thread accumulates the 28 FP32 terms (21 A^TA, 6 A^Tb, b^2)
-// over a grid-stride run of pixels; waves combine with xor-shuffle butterflies
-// and blocks through LDS.  In compensated mode each term is carried as a
+// over a grid-stride run of pixels; waves combine with DPP lane moves (quad
+// permutes, row rotations, row broadcasts) and blocks through LDS.  In compensated mode each term is carried as a
 // (sum, error) pair updated with the error-free TwoSum, so a block partial is
 // accurate to ~2^-48 relative; plain mode uses ordinary FP32 sums.  Blocks write
 // their partials, and the last block to arrive (device-scope counter) adds them
@@ -34,6 +34,7 @@ constexpr int kSlot = 64;    // floats per block partial: 28 (hi,lo) + 2 int cou
 constexpr int kThreads = 256;
 constexpr double kPivotRelEps = 1e-9;
 constexpr int kMinInliers = 6;
+constexpr int kFinUnroll = 12;  // partial-slot loads in flight per thread in the final sum
 
 struct IcpArgs {
   const float *lvx, *lvy, *lvz, *lnx, *lny, *lnz;  // live level maps (camera frame)
@@ -94,12 +95,26 @@ __device__ __forceinline__ int associate(const IcpArgs& a, const kf_posef& wc, c
   return 2;
 }
 
+// One DPP step of the wave reduction on a (sum, error) pair.
+template <int CTRL, int ROW_MASK, bool COMP>
+__device__ __forceinline__ void dpp_step(float& s, float& c, int = 0) {
+  const float so = dpp_f<CTRL, ROW_MASK>(s);
+  if (COMP) {
+    const float co = dpp_f<CTRL, ROW_MASK>(c);
+    float e = 0.0f;
+    two_sum(s, e, so);
+    c = (c + co) + e;
+  } else {
+    s = s + so;
+  }
+}
+
 template <bool COMP>
 __global__ __launch_bounds__(kThreads) void icp_iteration(IcpArgs a) {
   __shared__ float red_s[4][kTerms];
   __shared__ float red_c[4][kTerms];
   __shared__ int red_n[4][2];
-  __shared__ double fin[8][32];
+  __shared__ double fin[16][kTerms + 2];
   __shared__ int is_last;
   if (a.mode == 0 && a.ctrl->stop) return;  // uniform over the grid
 
@@ -138,26 +153,20 @@ __global__ __launch_bounds__(kThreads) void icp_iteration(IcpArgs a) {
     if (COMP) two_sum(s[27], c[27], b * b); else s[27] = s[27] + b * b;
   }
 
-  // Wave butterfly: every lane ends with the same (sum, error) pair.
+  // Wave reduction with DPP lane moves (no LDS round trips): the (sum, error)
+  // pairs of all 64 lanes end up in lane 63.
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-    for (int k = 0; k < kTerms; ++k) {
-      const float so = __shfl_xor(s[k], off, kWave);
-      if (COMP) {
-        const float co = __shfl_xor(c[k], off, kWave);
-        float t = s[k], e = 0.0f;
-        two_sum(t, e, so);
-        s[k] = t;
-        c[k] = (c[k] + co) + e;
-      } else {
-        s[k] = s[k] + so;
-      }
-    }
+  for (int k = 0; k < kTerms; ++k) {
+    dpp_step<0xB1, 0xf, COMP>(s[k], c[k]);
+    dpp_step<0x4E, 0xf, COMP>(s[k], c[k]);
+    dpp_step<0x124, 0xf, COMP>(s[k], c[k]);
+    dpp_step<0x128, 0xf, COMP>(s[k], c[k]);
+    dpp_step<0x142, 0xA, COMP>(s[k], c[k]);
+    dpp_step<0x143, 0xC, COMP>(s[k], c[k]);
   }
-  n_in = wave_sum_i(n_in);
-  n_cand = wave_sum_i(n_cand);
-  if (lane == 0) {
+  n_in = wave_sum_i_lane63(n_in);
+  n_cand = wave_sum_i_lane63(n_cand);
+  if (lane == 63) {
 #pragma unroll
     for (int k = 0; k < kTerms; ++k) {
       red_s[wave][k] = s[k];
@@ -200,23 +209,42 @@ __global__ __launch_bounds__(kThreads) void icp_iteration(IcpArgs a) {
   if (!is_last) return;
   __threadfence();
 
-  // FP64 cross-block sum: thread (g, k) adds blocks g, g+8, ... of term k in order.
-  const int k = tid & 31, g = tid >> 5;
-  double acc = 0.0;
-  if (k < kTerms) {
-    for (int bb = g; bb < (int)gridDim.x; bb += 8) {
-      const float* sl = a.partials + (size_t)bb * kSlot;
-      acc = (acc + (double)sl[2 * k]) + (double)sl[2 * k + 1];
+  // FP64 cross-block sum.  Thread (g, j) reads float4 j of the slots of blocks
+  // g, g + 16, ... (j < 14: terms 2j and 2j + 1 as (hi, lo) pairs; j = 14: the
+  // inlier and candidate counts) with kFinUnroll independent loads in flight,
+  // then the 16 group totals of each term are added in group order.
+  {
+    const int j = tid & 15, g = tid >> 4;
+    double a0 = 0.0, a1 = 0.0;
+    if (j < 15) {
+      const float4* base = reinterpret_cast<const float4*>(a.partials) + j;
+      const int nb = (int)gridDim.x;
+      for (int b0 = g; b0 < nb; b0 += 16 * kFinUnroll) {
+        float4 v[kFinUnroll];
+#pragma unroll
+        for (int u = 0; u < kFinUnroll; ++u) {
+          const int bb = b0 + 16 * u;
+          v[u] = bb < nb ? base[(size_t)bb * (kSlot / 4)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int u = 0; u < kFinUnroll; ++u) {
+          if (j < 14) {
+            a0 = (a0 + (double)v[u].x) + (double)v[u].y;
+            a1 = (a1 + (double)v[u].z) + (double)v[u].w;
+          } else {
+            a0 = a0 + (double)__float_as_int(v[u].x);
+            a1 = a1 + (double)__float_as_int(v[u].y);
+          }
+        }
+      }
+      fin[g][2 * j] = a0;
+      fin[g][2 * j + 1] = a1;
     }
-  } else if (k < kTerms + 2) {
-    for (int bb = g; bb < (int)gridDim.x; bb += 8)
-      acc = acc + (double)__float_as_int(a.partials[(size_t)bb * kSlot + 56 + (k - kTerms)]);
   }
-  fin[g][k] = acc;
   __syncthreads();
-  if (tid < 32) {
+  if (tid < kTerms + 2) {
     double t = fin[0][tid];
-    for (int gg = 1; gg < 8; ++gg) t = t + fin[gg][tid];
+    for (int gg = 1; gg < 16; ++gg) t = t + fin[gg][tid];
     fin[0][tid] = t;
   }
   __syncthreads();

@@ -202,9 +202,15 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
     a.out.ny[idx] = nrm.y;
     a.out.nz[idx] = nrm.z;
   }
-  if (a.steps) {
-    const int tot = wave_sum_i(nsteps);
-    if (lane == 0 && tot) atomicAdd(a.steps, (unsigned long long)tot);
+  if (a.steps) {  // one device atomic per block (uniform branch)
+    __shared__ int red[4];
+    const int tot = wave_sum_i_lane63(nsteps);
+    if (lane == 63) red[wave] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int b = (red[0] + red[1]) + (red[2] + red[3]);
+      if (b) atomicAdd(a.steps, (unsigned long long)b);
+    }
   }
 }
 

@@ -10,11 +10,11 @@
 //   1. column_spans: one thread per (x,y) column computes, in FP64, a conservative
 //      z-interval outside which no voxel can pass the exact per-voxel test (the
 //      column is a line in camera space, the frustum is convex).
-//   2. integrate: one wavefront per column walks its span with 16-byte vector
-//      accesses (2 FP32 or 4 FP16 voxels per lane), evaluates the exact FP32
-//      test per voxel (same expressions as the reference's per-voxel body, in
-//      metres and with a z-forward camera), and loads/stores a vector only if
-//      one of its voxels is updated.
+//   2. integrate: one wavefront per column (waves stride over the columns) walks
+//      its span with 16-byte vector accesses (2 FP32 or 4 FP16 voxels per lane),
+//      evaluates the exact FP32 test per voxel (same expressions as the
+//      reference's per-voxel body, in metres and with a z-forward camera), and
+//      loads/stores a vector only if one of its voxels is updated.
 #include "kf_device.hpp"
 #include "kf_internal.hpp"
 
@@ -22,6 +22,7 @@ namespace kf {
 namespace {
 
 constexpr int kWavesPerBlock = 4;
+constexpr int kSweepMaxBlocks = 4096;  // 16384 waves: two full waves of residency on 256 CUs
 
 struct SweepArgs {
   void* vox;
@@ -120,67 +121,75 @@ __device__ __forceinline__ bool voxel_sample(const SweepArgs& a, float ax, float
 template <int FMT>
 __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
   constexpr int V = (FMT == KF_VOXEL_F16) ? 4 : 2;  // voxels per 16-byte lane access
-  const int lane = threadIdx.x & (kWave - 1);
-  const int c = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (c >= a.n * a.n) return;
-  const uint32_t span = a.spans[c];
-  const int z0 = (int)(span & 0xffffu), z1 = (int)(span >> 16);
-  if (z1 < z0) return;
-  const int ix = c / a.n, iy = c - ix * a.n;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
   const kf_posef P = a.pose->cw;
-  const float wx = a.ox + ((float)ix + 0.5f) * a.h;
-  const float wy = a.oy + ((float)iy + 0.5f) * a.h;
-  const float ax = (P.r[0] * wx + P.r[1] * wy) + P.t[0];
-  const float ay = (P.r[3] * wx + P.r[4] * wy) + P.t[1];
-  const float az = (P.r[6] * wx + P.r[7] * wy) + P.t[2];
-  const size_t col = (size_t)c * (size_t)a.n;
+  const int ncol = a.n * a.n;
   int count = 0;
-  for (int zb = (z0 & ~(V - 1)) + lane * V; zb <= z1; zb += kWave * V) {
-    float fs[V];
-    unsigned upd = 0;
+  // Waves stride over the columns; neighbouring waves take neighbouring columns.
+  for (int c = blockIdx.x * kWavesPerBlock + wave; c < ncol; c += gridDim.x * kWavesPerBlock) {
+    const uint32_t span = a.spans[c];
+    const int z0 = (int)(span & 0xffffu), z1 = (int)(span >> 16);
+    if (z1 < z0) continue;
+    const int ix = c / a.n, iy = c - ix * a.n;
+    const float wx = a.ox + ((float)ix + 0.5f) * a.h;
+    const float wy = a.oy + ((float)iy + 0.5f) * a.h;
+    const float ax = (P.r[0] * wx + P.r[1] * wy) + P.t[0];
+    const float ay = (P.r[3] * wx + P.r[4] * wy) + P.t[1];
+    const float az = (P.r[6] * wx + P.r[7] * wy) + P.t[2];
+    const size_t col = (size_t)c * (size_t)a.n;
+    for (int zb = (z0 & ~(V - 1)) + lane * V; zb <= z1; zb += kWave * V) {
+      float fs[V];
+      unsigned upd = 0;
 #pragma unroll
-    for (int j = 0; j < V; ++j) {
-      fs[j] = 0.0f;
-      if (voxel_sample(a, ax, ay, az, P.r, zb + j, fs[j])) upd |= 1u << j;
-    }
-    if (!upd) continue;
-    count += __builtin_popcount(upd);
-    const size_t idx = col + (size_t)zb;
-    if (FMT == KF_VOXEL_F16) {
-      uint4* p = reinterpret_cast<uint4*>(static_cast<uint32_t*>(a.vox) + idx);
-      uint4 w4 = *p;
-      uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (upd & (1u << j)) {
-          const float fo = half_bits_to_float(w[j] & 0xffffu);
-          const float wo = (float)(w[j] >> 16);
-          // Free space (f_old = f = 1) averages to exactly 1: (w + 1) / (w + 1).
-          const float fn = (fo == 1.0f && fs[j] == 1.0f) ? 1.0f : (fo * wo + fs[j]) / (wo + 1.0f);
-          const float wn = fminf(wo + 1.0f, a.maxw);
-          w[j] = float_to_half_bits(fn) | ((uint32_t)wn << 16);
-        }
+      for (int j = 0; j < V; ++j) {
+        fs[j] = 0.0f;
+        if (voxel_sample(a, ax, ay, az, P.r, zb + j, fs[j])) upd |= 1u << j;
       }
-      *p = make_uint4(w[0], w[1], w[2], w[3]);
-    } else {
-      float4* p = reinterpret_cast<float4*>(static_cast<float2*>(a.vox) + idx);
-      float4 v4 = *p;
-      float fw[4] = {v4.x, v4.y, v4.z, v4.w};
+      if (!upd) continue;
+      count += __builtin_popcount(upd);
+      const size_t idx = col + (size_t)zb;
+      if (FMT == KF_VOXEL_F16) {
+        uint4* p = reinterpret_cast<uint4*>(static_cast<uint32_t*>(a.vox) + idx);
+        uint4 w4 = *p;
+        uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        if (upd & (1u << j)) {
-          const float fo = fw[2 * j], wo = fw[2 * j + 1];
-          fw[2 * j] = (fo == 1.0f && fs[j] == 1.0f) ? 1.0f : (fo * wo + fs[j]) / (wo + 1.0f);
-          fw[2 * j + 1] = fminf(wo + 1.0f, a.maxw);
+        for (int j = 0; j < 4; ++j) {
+          if (upd & (1u << j)) {
+            const float fo = half_bits_to_float(w[j] & 0xffffu);
+            const float wo = (float)(w[j] >> 16);
+            // Free space (f_old = f = 1) averages to exactly 1: (w + 1) / (w + 1).
+            const float fn = (fo == 1.0f && fs[j] == 1.0f) ? 1.0f : (fo * wo + fs[j]) / (wo + 1.0f);
+            const float wn = fminf(wo + 1.0f, a.maxw);
+            w[j] = float_to_half_bits(fn) | ((uint32_t)wn << 16);
+          }
         }
+        *p = make_uint4(w[0], w[1], w[2], w[3]);
+      } else {
+        float4* p = reinterpret_cast<float4*>(static_cast<float2*>(a.vox) + idx);
+        float4 v4 = *p;
+        float fw[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          if (upd & (1u << j)) {
+            const float fo = fw[2 * j], wo = fw[2 * j + 1];
+            fw[2 * j] = (fo == 1.0f && fs[j] == 1.0f) ? 1.0f : (fo * wo + fs[j]) / (wo + 1.0f);
+            fw[2 * j + 1] = fminf(wo + 1.0f, a.maxw);
+          }
+        }
+        *p = make_float4(fw[0], fw[1], fw[2], fw[3]);
       }
-      *p = make_float4(fw[0], fw[1], fw[2], fw[3]);
+      if (a.mask) atomicOr(&a.mask[idx >> 5], upd << (idx & 31));
     }
-    if (a.mask) atomicOr(&a.mask[idx >> 5], upd << (idx & 31));
   }
-  if (a.updated) {
-    count = wave_sum_i(count);
-    if (lane == 0 && count) atomicAdd(a.updated, (unsigned long long)count);
+  if (a.updated) {  // one device atomic per block (uniform branch)
+    __shared__ int red[kWavesPerBlock];
+    count = wave_sum_i_lane63(count);
+    if (lane == 63) red[wave] = count;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int tot = (red[0] + red[1]) + (red[2] + red[3]);
+      if (tot) atomicAdd(a.updated, (unsigned long long)tot);
+    }
   }
 }
 
@@ -304,7 +313,7 @@ kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integ
   hipLaunchKernelGGL(column_spans, dim3(div_up(cols, 256)), dim3(256), 0, s, a);
   KF_TRY(check_launch("column_spans"));
   if (between) KF_HIP(hipEventRecord(between, s));
-  const int blocks = div_up(cols, kWavesPerBlock);
+  const int blocks = std::min(div_up(cols, kWavesPerBlock), kSweepMaxBlocks);
   if (d.format == KF_VOXEL_F16)
     hipLaunchKernelGGL(integrate<KF_VOXEL_F16>, dim3(blocks), dim3(256), 0, s, a);
   else

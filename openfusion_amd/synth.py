@@ -11,7 +11,7 @@ Presets (volume, truncation, scene, trajectory and seeds):
 * ``prior``   — ``full16`` stream for the learned prior (training trajectories:
   seeds 10000-10199, <= 6 cm / 4 deg per frame, 101 frames each).
 * ``stress``  — 768^3 FP16 over 6 m, T 0.03 m; 1000 frames; 6x3x6 m room with 20
-  primitives; bursts up to 6 cm / 5 deg per frame.
+  primitives; bursts up to 6 cm / 5 deg per frame (``stress32``: FP32 volume).
 
 All share K = (525, 525, 319.5, 239.5) at 640x480, depth range [0.3, 4] m,
 ICP 3 levels 10/5/4 (finest first) with 0.1 m / 20 deg gates.
@@ -117,6 +117,7 @@ PRESETS = {
                      traj=KF_TRAJ_BURST, max_trans=0.06, max_rot=5.0, scene_seed=1004, traj_seed=3004,
                      noise_seed=2004, margin=0.8),
 }
+PRESETS["stress32"] = dataclasses.replace(PRESETS["stress"], name="stress32", fmt=kf.KF_VOXEL_F32)
 
 
 def intrinsics() -> kf.Intrinsics:
