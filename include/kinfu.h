@@ -145,7 +145,21 @@ kf_status kf_volume_create(const kf_volume_desc* desc, kf_volume** out);
 kf_status kf_volume_destroy(kf_volume* vol);
 kf_status kf_volume_reset(kf_volume* vol, kf_stream s);
 kf_status kf_volume_get_desc(const kf_volume* vol, kf_volume_desc* desc);
+/* Raw voxels (layout: voxel (x,y,z) at (x*n + y)*n + z, 8-byte {f32 f, f32 w} or
+ * 4-byte {f16 f, u16 w}, f = tsdf / T).  Handing out the pointer marks the brick
+ * occupancy map stale (the next raycast rebuilds it); a caller that keeps the pointer
+ * and writes through it later calls kf_volume_mark_modified before raycasting. */
 kf_status kf_volume_data(const kf_volume* vol, void** d_voxels, size_t* bytes);
+/* Read-only view of the same voxels (leaves the occupancy map as it is). */
+kf_status kf_volume_data_const(const kf_volume* vol, const void** d_voxels, size_t* bytes);
+kf_status kf_volume_mark_modified(kf_volume* vol);
+/* Brick occupancy map used by the raycaster to skip free space (volumes with
+ * n <= 512; bricks_per_axis = n / 8, else 0 and no map): word bx*nb + by, bit bz
+ * clear => every voxel (x,y,z) with x in [8bx, 8bx+8], y in [8by, 8by+8], z in
+ * [8bz, 8bz+8] has f == 1.  No counterpart in the reference (its raycaster marches
+ * every step).  Rebuilds a stale map on stream s first. */
+kf_status kf_volume_occupancy(const kf_volume* vol, const uint64_t** d_words, int32_t* bricks_per_axis,
+                              kf_stream s);
 /* Occupancy statistics over every voxel (reads the whole volume once):
  * d_out (device, 4 doubles) = {observed voxels (w > 0), sum of w, sum of f over
  * observed voxels, voxels with f < 0 and w > 0}. */

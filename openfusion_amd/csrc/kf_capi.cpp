@@ -268,6 +268,16 @@ extern "C" kf_status kf_volume_create(const kf_volume_desc* d, kf_volume** out) 
     delete v;
     return hip_fail(e, "hipMalloc(spans)");
   }
+  if (d->n / 8 <= kf::kOccMaxBricks) {
+    v->occ_nb = d->n / 8;
+    e = hipMalloc(&v->d_occ, (size_t)v->occ_nb * v->occ_nb * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+      (void)hipFree(v->d_voxels);
+      (void)hipFree(v->d_spans);
+      delete v;
+      return hip_fail(e, "hipMalloc(occupancy)");
+    }
+  }
   kf_status st = launch_volume_reset(v, nullptr);
   if (st == KF_OK) {
     e = hipDeviceSynchronize();
@@ -276,6 +286,7 @@ extern "C" kf_status kf_volume_create(const kf_volume_desc* d, kf_volume** out) 
   if (st != KF_OK) {
     (void)hipFree(v->d_voxels);
     (void)hipFree(v->d_spans);
+    (void)hipFree(v->d_occ);
     delete v;
     return st;
   }
@@ -287,6 +298,7 @@ extern "C" kf_status kf_volume_destroy(kf_volume* v) {
   if (!v) return KF_OK;
   (void)hipFree(v->d_voxels);
   (void)hipFree(v->d_spans);
+  (void)hipFree(v->d_occ);
   delete v;
   return KF_OK;
 }
@@ -306,6 +318,29 @@ extern "C" kf_status kf_volume_data(const kf_volume* v, void** p, size_t* bytes)
   KF_CHECK_ARG(v && p && bytes, "NULL argument");
   *p = v->d_voxels;
   *bytes = v->bytes;
+  v->occ_stale = true;  // the caller may write voxels: the next raycast rebuilds the brick map
+  return KF_OK;
+}
+
+extern "C" kf_status kf_volume_data_const(const kf_volume* v, const void** p, size_t* bytes) {
+  KF_CHECK_ARG(v && p && bytes, "NULL argument");
+  *p = v->d_voxels;
+  *bytes = v->bytes;
+  return KF_OK;
+}
+
+extern "C" kf_status kf_volume_mark_modified(kf_volume* v) {
+  KF_CHECK_ARG(v != nullptr, "volume is NULL");
+  v->occ_stale = true;
+  return KF_OK;
+}
+
+extern "C" kf_status kf_volume_occupancy(const kf_volume* v, const uint64_t** d_words, int32_t* bricks_per_axis,
+                                         kf_stream s) {
+  KF_CHECK_ARG(v && d_words && bricks_per_axis, "NULL argument");
+  KF_TRY(kf::ensure_occupancy(v, S(s)));
+  *d_words = reinterpret_cast<const uint64_t*>(v->d_occ);
+  *bricks_per_axis = v->occ_nb;
   return KF_OK;
 }
 

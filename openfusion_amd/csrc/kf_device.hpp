@@ -109,6 +109,51 @@ __device__ __forceinline__ int wave_sum_i_lane63(int v) {
   return v;
 }
 
+// Bitwise OR over the 64 lanes of a wavefront (DPP ladder), returned to all lanes.
+__device__ __forceinline__ unsigned wave_or_u32(unsigned v) {
+  int x = (int)v;
+  x |= dpp_i<0xB1>(x);
+  x |= dpp_i<0x4E>(x);
+  x |= dpp_i<0x124>(x);
+  x |= dpp_i<0x128>(x);
+  x |= dpp_i<0x142, 0xA>(x);
+  x |= dpp_i<0x143, 0xC>(x);
+  return (unsigned)__builtin_amdgcn_readlane(x, 63);
+}
+
+// Brick occupancy (bricks of 8^3 voxels).  A voxel at index z along an axis lies in
+// the extended extent [8b, 8b + 8] of brick b = z >> 3 and, when z is a multiple of
+// 8, of brick b - 1 as well.
+__device__ __forceinline__ unsigned long long occ_zbits(int z) {
+  const int b = z >> 3;
+  unsigned long long m = 1ull << b;
+  if ((z & 7) == 0 && b > 0) m |= 1ull << (b - 1);
+  return m;
+}
+
+// Marks, for voxel column (ix, iy), the z-bricks in the lanes' bit sets (whole
+// wave, uniform control flow).  Up to four brick columns receive the bits (the
+// column's own and, on a brick boundary, the one below in x and/or y); a word is
+// only written when bits are missing.
+__device__ __forceinline__ void occ_mark_column(unsigned long long* occ, int nb, int ix, int iy,
+                                                unsigned long long bits, int lane) {
+  if (__ballot(bits != 0ull) == 0ull) return;
+  const unsigned lo = wave_or_u32((unsigned)bits);
+  const unsigned hi = wave_or_u32((unsigned)(bits >> 32));
+  const unsigned long long zm = ((unsigned long long)hi << 32) | lo;
+  if (lane < 4) {
+    const int bx0 = ix >> 3, by0 = iy >> 3;
+    const int bx = (lane & 1) ? (((ix & 7) == 0 && bx0 > 0) ? bx0 - 1 : -1) : bx0;
+    const int by = (lane & 2) ? (((iy & 7) == 0 && by0 > 0) ? by0 - 1 : -1) : by0;
+    if (bx >= 0 && by >= 0) {
+      unsigned long long* w = occ + (bx * nb + by);
+      const unsigned long long cur = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((cur & zm) != zm)
+        __hip_atomic_fetch_or(w, zm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 // Wave-level sum of a float with the DPP/permute ladder, result valid in all lanes.
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll

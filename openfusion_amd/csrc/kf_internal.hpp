@@ -47,6 +47,14 @@ struct kf_volume {
   void* d_voxels = nullptr;  // n^3 * voxel_bytes
   size_t bytes = 0;
   uint32_t* d_spans = nullptr;  // per (x,y) column: z0 | z1 << 16, conservative frustum span
+  // Brick occupancy map (volumes with n <= 512): bricks of 8^3 voxels, one 64-bit
+  // word per (bx, by) brick column with bit bz.  A clear bit guarantees f == 1 for
+  // every voxel of the brick extended by one voxel along +x, +y, +z (all corners
+  // of a trilinear sample whose base voxel lies in the brick).  The sweep sets
+  // bits as it writes f != 1; bits are never cleared except by a reset or a rebuild.
+  unsigned long long* d_occ = nullptr;
+  int occ_nb = 0;                // bricks per axis (0: no map)
+  mutable bool occ_stale = false;  // voxels may have been written outside the library
 };
 
 // ICP scratch and per-iteration control block (device memory).
@@ -79,6 +87,9 @@ kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integ
                            unsigned long long* d_updated, uint32_t* d_mask, hipStream_t s,
                            hipEvent_t between = nullptr);
 kf_status launch_volume_stats(const kf_volume* v, double* d_out, hipStream_t s);
+// Recomputes the occupancy map from the voxels when it is stale (no-op otherwise).
+kf_status ensure_occupancy(const kf_volume* v, hipStream_t s);
+constexpr int kOccMaxBricks = 64;  // bricks per axis held in one 64-bit word
 kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_raycast_params& p,
                          const kf_pose* d_pose, kf_frame_maps& model, unsigned long long* d_steps,
                          hipStream_t s);

@@ -416,6 +416,9 @@ extern "C" kf_status kf_pipeline_process(kf_pipeline* p, const uint16_t* d_depth
   hipStream_t s = p->stream;
   const size_t bytes = (size_t)p->d.intr.width * p->d.intr.height * sizeof(uint16_t);
   KF_HIP(hipMemcpyAsync(p->d_depth, d_depth, bytes, hipMemcpyDeviceToDevice, s));
+  // Voxels written outside the library since the last frame: rebuild the brick map
+  // here, eagerly, so that it is never recorded into the frame graph.
+  KF_TRY(ensure_occupancy(p->vol, s));
   if (p->frame == 0) {
     KF_TRY(bootstrap_frame(p));
   } else if (p->stage_timing || !p->d.use_graph) {

@@ -25,6 +25,8 @@ struct RayArgs {
   const kf_pose* pose;
   kf_level_maps out;
   unsigned long long* steps;
+  const unsigned long long* occ;  // brick occupancy map or NULL
+  int occ_nb;
   int n;
   float h, inv_h, ox, oy, oz;
   float fx, fy, cx, cy;
@@ -44,6 +46,9 @@ __device__ __forceinline__ float vox_f(const void* vox, size_t i) {
 __device__ __forceinline__ float lerp(float a, float b, float t) { return a + (b - a) * t; }
 
 // Trilinear sample of f at world point p; false outside the voxel-centre domain.
+// When the brick of the base voxel is clear in the occupancy map, all eight
+// corners hold f == 1 and the interpolation is exactly 1 (a + (b - a) t with
+// a = b = 1), so the voxel loads are skipped: the value is the same.
 template <int FMT>
 __device__ __forceinline__ bool sample(const RayArgs& a, f3 p, float& out) {
   const float gx = (p.x - a.ox) * a.inv_h - 0.5f;
@@ -52,6 +57,13 @@ __device__ __forceinline__ bool sample(const RayArgs& a, f3 p, float& out) {
   const float lim = (float)(a.n - 1);
   if (!(gx >= 0.0f && gx < lim && gy >= 0.0f && gy < lim && gz >= 0.0f && gz < lim)) return false;
   const int ix = (int)gx, iy = (int)gy, iz = (int)gz;
+  if (a.occ) {
+    const unsigned long long w = a.occ[(ix >> 3) * a.occ_nb + (iy >> 3)];
+    if (!((w >> (iz >> 3)) & 1ull)) {
+      out = 1.0f;
+      return true;
+    }
+  }
   const float tx = gx - (float)ix, ty = gy - (float)iy, tz = gz - (float)iz;
   const size_t n = (size_t)a.n;
   const size_t b = ((size_t)ix * n + (size_t)iy) * n + (size_t)iz;
@@ -261,6 +273,9 @@ kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_r
   a.pose = d_pose;
   a.out = model.level[0];
   a.steps = d_steps;
+  KF_TRY(ensure_occupancy(v, s));
+  a.occ = v->d_occ;
+  a.occ_nb = v->occ_nb;
   a.n = d.n;
   a.h = d.extent_m / (float)d.n;
   a.inv_h = (float)d.n / d.extent_m;

@@ -31,6 +31,8 @@ struct SweepArgs {
   const kf_pose* pose;
   unsigned long long* updated;
   uint32_t* mask;
+  unsigned long long* occ;  // brick occupancy map or NULL
+  int occ_nb;
   int n;
   float h, ox, oy, oz;
   float fx, fy, cx, cy;
@@ -137,6 +139,7 @@ __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
     const float ay = (P.r[3] * wx + P.r[4] * wy) + P.t[1];
     const float az = (P.r[6] * wx + P.r[7] * wy) + P.t[2];
     const size_t col = (size_t)c * (size_t)a.n;
+    unsigned long long occ_bits = 0ull;  // z-bricks receiving f != 1 from this lane
     for (int zb = (z0 & ~(V - 1)) + lane * V; zb <= z1; zb += kWave * V) {
       float fs[V];
       unsigned upd = 0;
@@ -161,6 +164,7 @@ __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
             const float fn = (fo == 1.0f && fs[j] == 1.0f) ? 1.0f : (fo * wo + fs[j]) / (wo + 1.0f);
             const float wn = fminf(wo + 1.0f, a.maxw);
             w[j] = float_to_half_bits(fn) | ((uint32_t)wn << 16);
+            if (a.occ && (w[j] & 0xffffu) != 0x3C00u) occ_bits |= occ_zbits(zb + j);
           }
         }
         *p = make_uint4(w[0], w[1], w[2], w[3]);
@@ -174,12 +178,14 @@ __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
             const float fo = fw[2 * j], wo = fw[2 * j + 1];
             fw[2 * j] = (fo == 1.0f && fs[j] == 1.0f) ? 1.0f : (fo * wo + fs[j]) / (wo + 1.0f);
             fw[2 * j + 1] = fminf(wo + 1.0f, a.maxw);
+            if (a.occ && fw[2 * j] != 1.0f) occ_bits |= occ_zbits(zb + j);
           }
         }
         *p = make_float4(fw[0], fw[1], fw[2], fw[3]);
       }
       if (a.mask) atomicOr(&a.mask[idx >> 5], upd << (idx & 31));
     }
+    if (a.occ) occ_mark_column(a.occ, a.occ_nb, ix, iy, occ_bits, lane);
   }
   if (a.updated) {  // one device atomic per block (uniform branch)
     __shared__ int red[kWavesPerBlock];
@@ -242,6 +248,39 @@ __global__ __launch_bounds__(256) void volume_stats(const uint4* p, size_t n16, 
   if (threadIdx.x < 4) atomicAdd(&out[threadIdx.x], red[threadIdx.x][0]);
 }
 
+// Occupancy from scratch: one wavefront per voxel column (waves stride over the
+// columns) reads the whole column with 16-byte loads and marks the z-bricks that
+// hold f != 1.
+template <int FMT>
+__global__ __launch_bounds__(256) void occupancy_rebuild(const void* vox, int n, unsigned long long* occ, int nb) {
+  constexpr int V = (FMT == KF_VOXEL_F16) ? 4 : 2;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+  const int ncol = n * n;
+  for (int c = blockIdx.x * kWavesPerBlock + wave; c < ncol; c += gridDim.x * kWavesPerBlock) {
+    const int ix = c / n, iy = c - ix * n;
+    const size_t col = (size_t)c * (size_t)n;
+    unsigned long long bits = 0ull;
+    for (int zb = lane * V; zb < n; zb += kWave * V) {
+      const uint4 q = *reinterpret_cast<const uint4*>(static_cast<const uint32_t*>(vox) + (FMT == KF_VOXEL_F16 ? col + zb : 2 * (col + zb)));
+      if (FMT == KF_VOXEL_F16) {
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if ((w[j] & 0xffffu) != 0x3C00u) bits |= occ_zbits(zb + j);
+      } else {
+        if (__uint_as_float(q.x) != 1.0f) bits |= occ_zbits(zb);
+        if (__uint_as_float(q.z) != 1.0f) bits |= occ_zbits(zb + 1);
+      }
+    }
+    occ_mark_column(occ, nb, ix, iy, bits, lane);
+  }
+}
+
+__global__ __launch_bounds__(256) void fill_u64(unsigned long long* p, int count, unsigned long long v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
 __global__ void zero_doubles(double* p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0.0;
 }
@@ -276,7 +315,32 @@ kf_status launch_volume_reset(kf_volume* v, hipStream_t s) {
   const int blocks = (int)std::min<size_t>((n16 + 255) / 256, 8192);
   hipLaunchKernelGGL(volume_fill, dim3(blocks), dim3(256), 0, s, static_cast<uint4*>(v->d_voxels), n16,
                      word);
-  return check_launch("volume_fill");
+  KF_TRY(check_launch("volume_fill"));
+  if (v->d_occ) {  // every voxel is free space: all bricks clear
+    const int words = v->occ_nb * v->occ_nb;
+    hipLaunchKernelGGL(fill_u64, dim3(div_up(words, 256)), dim3(256), 0, s, v->d_occ, words, 0ull);
+    KF_TRY(check_launch("fill_u64"));
+  }
+  v->occ_stale = false;
+  return KF_OK;
+}
+
+kf_status ensure_occupancy(const kf_volume* v, hipStream_t s) {
+  if (!v->d_occ || !v->occ_stale) return KF_OK;
+  const int words = v->occ_nb * v->occ_nb;
+  hipLaunchKernelGGL(fill_u64, dim3(div_up(words, 256)), dim3(256), 0, s, v->d_occ, words, 0ull);
+  KF_TRY(check_launch("fill_u64"));
+  const int n = v->desc.n;
+  const int blocks = std::min(div_up(n * n, kWavesPerBlock), kSweepMaxBlocks);
+  if (v->desc.format == KF_VOXEL_F16)
+    hipLaunchKernelGGL(occupancy_rebuild<KF_VOXEL_F16>, dim3(blocks), dim3(256), 0, s,
+                       (const void*)v->d_voxels, n, v->d_occ, v->occ_nb);
+  else
+    hipLaunchKernelGGL(occupancy_rebuild<KF_VOXEL_F32>, dim3(blocks), dim3(256), 0, s,
+                       (const void*)v->d_voxels, n, v->d_occ, v->occ_nb);
+  KF_TRY(check_launch("occupancy_rebuild"));
+  v->occ_stale = false;
+  return KF_OK;
 }
 
 kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integrate_params& p,
@@ -293,6 +357,8 @@ kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integ
   a.pose = d_pose;
   a.updated = d_updated;
   a.mask = d_mask;
+  a.occ = v->d_occ;
+  a.occ_nb = v->occ_nb;
   a.n = d.n;
   a.h = d.extent_m / (float)d.n;
   a.ox = d.origin_m[0];

@@ -142,6 +142,9 @@ _PROTOS = {
     "kf_volume_get_desc": (C.c_int, [_VP, C.POINTER(VolumeDesc)]),
     "kf_volume_data": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(C.c_size_t)]),
     "kf_volume_stats": (C.c_int, [_VP, _VP, _VP]),
+    "kf_volume_mark_modified": (C.c_int, [_VP]),
+    "kf_volume_data_const": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(C.c_size_t)]),
+    "kf_volume_occupancy": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(C.c_int32), _VP]),
     "kf_volume_export_tsdf": (C.c_int, [_VP, _VP, _VP]),
     "kf_volume_extract_mesh": (C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP]),
     "kf_depth_median": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, C.c_uint16, _VP]),
@@ -437,13 +440,31 @@ class Volume:
         return p.value, nb.value
 
     def download_raw(self) -> np.ndarray:
-        p, nb = self.data_ptr()
-        return download_ptr(p, np.uint8, nb)
+        p = _VP()
+        nb = C.c_size_t()
+        check(lib().kf_volume_data_const(self.handle, C.byref(p), C.byref(nb)), "kf_volume_data_const")
+        return download_ptr(p.value, np.uint8, nb.value)
 
     def upload_raw(self, raw: np.ndarray) -> None:
         p, nb = self.data_ptr()
         assert raw.nbytes == nb
         upload_ptr(p, raw)
+
+    def mark_modified(self) -> None:
+        check(lib().kf_volume_mark_modified(self.handle), "kf_volume_mark_modified")
+
+    def occupancy(self) -> np.ndarray:
+        """Brick occupancy map as (nb, nb, nb) bool [bx, by, bz] (True: the brick may
+        hold f != 1); empty when the volume has no map (n > 512)."""
+        p = _VP()
+        nb = C.c_int32()
+        check(lib().kf_volume_occupancy(self.handle, C.byref(p), C.byref(nb), None), "kf_volume_occupancy")
+        synchronize()
+        if nb.value == 0:
+            return np.zeros((0, 0, 0), bool)
+        words = download_ptr(p.value, np.uint64, nb.value * nb.value).reshape(nb.value, nb.value)
+        bits = (words[:, :, None] >> np.arange(nb.value, dtype=np.uint64)[None, None, :]) & np.uint64(1)
+        return bits.astype(bool)
 
     def stats(self) -> np.ndarray:
         """[observed voxels, sum w, sum f (observed), observed voxels with f < 0]."""
