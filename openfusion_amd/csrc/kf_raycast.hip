@@ -149,61 +149,48 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
   if (active && hit && tnear < tfar) {
     // The march: step_far while the last sample is +1 (free space), step_near
     // otherwise; stop at the first sample < 0, outside the domain, past tfar or
-    // after max_steps.  In free space kSpec consecutive samples are fetched
-    // together (their positions are the sequential ones, t += step_far) and then
-    // consumed in order, so the result and the step count equal the sequential
-    // march while the dependent-load chain is kSpec times shorter.
+    // after max_steps.  Samples are fetched kSpec at a time assuming the step
+    // size stays the same (positions are the sequential ones, t += step) and then
+    // consumed in order; a batch ends where the sequential march would change
+    // step size or stop, so the result and the step count equal the sequential
+    // march while the dependent-load chain is up to kSpec times shorter.
     constexpr int kSpec = 4;
     float t = tnear, fp;
     bool ok = sample<FMT>(a, f3{o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z}, fp) && fp >= 0.0f;
     int s = 0;
     while (ok && s < a.max_steps) {
-      if (fp >= 1.0f) {
-        float tk[kSpec], fk[kSpec];
-        bool vk[kSpec];
-        float tt = t;
+      const bool far = fp >= 1.0f;
+      const float st = far ? a.step_far_m : a.step_near_m;
+      float tk[kSpec], fk[kSpec];
+      bool vk[kSpec];
+      float tt = t;
 #pragma unroll
-        for (int j = 0; j < kSpec; ++j) {
-          tt = tt + a.step_far_m;
-          tk[j] = tt;
-          fk[j] = 0.0f;
-          vk[j] = tk[j] <= tfar &&
-                  sample<FMT>(a, f3{o.x + tt * dir.x, o.y + tt * dir.y, o.z + tt * dir.z}, fk[j]);
-        }
-        bool done = false;
+      for (int j = 0; j < kSpec; ++j) {
+        tt = tt + st;
+        tk[j] = tt;
+        fk[j] = 0.0f;
+        vk[j] = tt <= tfar && sample<FMT>(a, f3{o.x + tt * dir.x, o.y + tt * dir.y, o.z + tt * dir.z}, fk[j]);
+      }
+      bool done = false;
 #pragma unroll
-        for (int j = 0; j < kSpec; ++j) {
-          if (s >= a.max_steps || !(tk[j] <= tfar) || !vk[j]) {
-            if (s < a.max_steps) ++nsteps;  // this step was taken and ended the march
-            done = true;
-            break;
-          }
-          ++nsteps;
-          ++s;
-          if (fk[j] < 0.0f) {
-            hit_at(t, a.step_far_m, fp, fk[j]);
-            done = true;
-            break;
-          }
-          t = tk[j];
-          fp = fk[j];
-          if (fp < 1.0f) break;  // the next step is a near step
-        }
-        if (done) break;
-      } else {
-        ++nsteps;
-        ++s;
-        const float tn = t + a.step_near_m;
-        if (tn > tfar) break;
-        float fc;
-        if (!sample<FMT>(a, f3{o.x + tn * dir.x, o.y + tn * dir.y, o.z + tn * dir.z}, fc)) break;
-        if (fc < 0.0f) {
-          hit_at(t, a.step_near_m, fp, fc);
+      for (int j = 0; j < kSpec; ++j) {
+        if (s >= a.max_steps || !vk[j]) {
+          if (s < a.max_steps) ++nsteps;  // this step was taken and ended the march
+          done = true;
           break;
         }
-        t = tn;
-        fp = fc;
+        ++nsteps;
+        ++s;
+        if (fk[j] < 0.0f) {
+          hit_at(t, st, fp, fk[j]);
+          done = true;
+          break;
+        }
+        t = tk[j];
+        fp = fk[j];
+        if ((fp >= 1.0f) != far) break;  // the next step has the other size
       }
+      if (done) break;
     }
   }
   if (active) {
