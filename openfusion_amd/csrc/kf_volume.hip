@@ -39,6 +39,7 @@ struct SweepArgs {
   int W, H;
   float dmin, dmax;
   float trunc, inv_trunc, maxw;
+  float band;  // T + max(0.01 T, 0.1 mm): half-width of the band where the square root is taken
 };
 
 __global__ __launch_bounds__(256) void volume_fill(uint4* p, size_t n16, uint4 word) {
@@ -113,7 +114,19 @@ __device__ __forceinline__ bool voxel_sample(const SweepArgs& a, float ax, float
   if (!(ru >= 2.0f && ru < (float)(a.W - 2) && rv >= 2.0f && rv < (float)(a.H - 2))) return false;
   const float rl = a.raylen[(int)rv * a.W + (int)ru];
   if (!(rl > 0.0f)) return false;
-  const float dist = sqrtf((px * px + py * py) + pz * pz);
+  const float d2 = (px * px + py * py) + pz * pz;
+  // Outside |rl - dist| <= band the outcome follows from d2 alone: the margin
+  // band - T >= max(0.01 T, 0.1 mm) is far above the FP32 error of d2, lo^2, hi^2,
+  // dist and sdf (< 2 um for rays up to 10 m), so free space yields exactly f = 1
+  // (sdf / T > 1 clamps to 1) and the far side is exactly "not updated" (sdf < -T),
+  // as the full expression gives.
+  const float lo = rl - a.band, hi = rl + a.band;
+  if (lo > 0.0f && d2 < lo * lo) {
+    f = 1.0f;
+    return true;
+  }
+  if (d2 > hi * hi) return false;
+  const float dist = sqrtf(d2);
   const float sdf = rl - dist;
   if (sdf < -a.trunc) return false;
   f = fmaxf(fminf(sdf * a.inv_trunc, 1.0f), -1.0f);
@@ -374,6 +387,7 @@ kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integ
   a.dmax = p.max_depth_m;
   a.trunc = d.trunc_m;
   a.inv_trunc = 1.0f / d.trunc_m;
+  a.band = d.trunc_m + std::max(0.01f * d.trunc_m, 1e-4f);
   a.maxw = d.max_weight;
   const int cols = d.n * d.n;
   hipLaunchKernelGGL(column_spans, dim3(div_up(cols, 256)), dim3(256), 0, s, a);

@@ -84,8 +84,9 @@ def voxel_centres(d: kf.VolumeDesc) -> np.ndarray:
 
 def fill_volume(d: kf.VolumeDesc, f: np.ndarray, w: np.ndarray | float = 1.0) -> np.ndarray:
     """Raw voxel bytes (kf layout) from tsdf/T and weights (n,n,n)."""
-    f = np.asarray(f, np.float32).reshape(-1)
+    f = np.asarray(f, np.float32)
     w = np.broadcast_to(np.asarray(w, np.float32), f.shape).reshape(-1)
+    f = f.reshape(-1)
     if d.format == kf.KF_VOXEL_F16:
         lo = f.astype(np.float16).view(np.uint16).astype(np.uint32)
         hi = w.astype(np.uint32) << 16
@@ -96,8 +97,9 @@ def fill_volume(d: kf.VolumeDesc, f: np.ndarray, w: np.ndarray | float = 1.0) ->
 
 
 def f64_volume(f: np.ndarray, w: np.ndarray | float = 1.0) -> np.ndarray:
-    f = np.asarray(f, np.float64).reshape(-1)
+    f = np.asarray(f, np.float64)
     w = np.broadcast_to(np.asarray(w, np.float64), f.shape).reshape(-1)
+    f = f.reshape(-1)
     fw = np.empty((f.size, 2), np.float64)
     fw[:, 0], fw[:, 1] = f, w
     return fw.reshape(-1).view(np.uint8)

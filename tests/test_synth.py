@@ -129,3 +129,14 @@ def test_presets_render(synth):
     assert s.depth.shape == (30, 480, 640)
     assert s.scene.count == 5
     assert (s.depth > 0).mean() > 0.9
+
+
+def test_fill_volume_layouts():
+    from tests.helpers import fill_volume, vol_desc
+    f = np.linspace(-1, 1, 8 ** 3, dtype=np.float32).reshape(8, 8, 8)
+    w = np.arange(8 ** 3, dtype=np.float32).reshape(8, 8, 8) % 7
+    raw32 = fill_volume(vol_desc(8, fmt=0), f, w).view(np.float32).reshape(-1, 2)
+    assert np.array_equal(raw32[:, 0], f.reshape(-1)) and np.array_equal(raw32[:, 1], w.reshape(-1))
+    raw16 = fill_volume(vol_desc(8, fmt=1), f, 3.0).view(np.uint32)
+    assert np.array_equal((raw16 & 0xFFFF).astype(np.uint16).view(np.float16), f.reshape(-1).astype(np.float16))
+    assert np.all(raw16 >> 16 == 3)
