@@ -248,12 +248,20 @@ __global__ __launch_bounds__(kThreads) void icp_iteration(IcpArgs a) {
     fin[0][tid] = t;
   }
   __syncthreads();
-  if (tid != 0) return;
+  // Tail: the solve runs on one lane; the Euler sines/cosines (3 lanes), the pose
+  // product (16 lanes) and the FP32 pose fields (24 lanes) run in parallel, each
+  // entry with the same expression as the serial helpers, so the bits are the same.
   const double* sum = fin[0];
-  const double inl = sum[28], cand = sum[29];
   if (a.mode == 1) {
-    for (int q = 0; q < 30; ++q) a.sums_out[q] = sum[q];
-  } else {
+    if (tid < kTerms + 2) a.sums_out[tid] = sum[tid];
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&a.ctrl->arrive, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  __shared__ double tx[6], tcs[6], tD[16], tT[16], tN[16];
+  __shared__ int tok;
+  if (tid == 0) {
+    const double inl = sum[28], cand = sum[29];
     kf_icp_stats* st = a.stats;
     const int ni = (int)inl;
     st->inliers = ni;
@@ -263,25 +271,47 @@ __global__ __launch_bounds__(kThreads) void icp_iteration(IcpArgs a) {
     st->level_inliers[a.level] = ni;
     double x[6];
     const bool ok = ni >= kMinInliers && cholesky_solve6(sum, sum + 21, kPivotRelEps, x);
-    if (ok) {
-      double D[16], Tn[16];
-      euler_zyx(-x[0], -x[1], -x[2], x[3], x[4], x[5], D);
-      mat4_mul(D, a.pose->T_wc, Tn);
-      kf_pose np;
-      for (int q = 0; q < 16; ++q) np.T_wc[q] = Tn[q];
-      np.T_wc[12] = 0.0;
-      np.T_wc[13] = 0.0;
-      np.T_wc[14] = 0.0;
-      np.T_wc[15] = 1.0;
-      pose_finish_fields(&np);
-      *a.pose = np;
-      a.ctrl->executed = a.ctrl->executed + 1;
-    } else {
+    tok = ok ? 1 : 0;
+    for (int q = 0; q < 6; ++q) tx[q] = ok ? x[q] : 0.0;
+    if (!ok) {
       a.ctrl->stop = 1;
       st->status = 1;
     }
+  } else if (tid >= 64 && tid < 80) {
+    tT[tid - 64] = a.pose->T_wc[tid - 64];
   }
-  __hip_atomic_store(&a.ctrl->arrive, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (tok) {  // uniform
+    if (tid < 3) {
+      const double ang = -tx[tid];  // rotation increment is the negated solution
+      tcs[2 * tid] = cos(ang);
+      tcs[2 * tid + 1] = sin(ang);
+    }
+    __syncthreads();
+    if (tid < 16) tD[tid] = euler_zyx_entry(tid, tcs[0], tcs[1], tcs[2], tcs[3], tcs[4], tcs[5], tx[3], tx[4], tx[5]);
+    __syncthreads();
+    if (tid < 16) {
+      const int i = tid >> 2, j = tid & 3;
+      tN[tid] = (i == 3) ? (j == 3 ? 1.0 : 0.0) : mat4_mul_entry(tD, tT, i, j);
+    }
+    __syncthreads();
+    kf_pose* P = a.pose;
+    if (tid < 16) {
+      P->T_wc[tid] = tN[tid];
+    } else if (tid >= 64 && tid < 76) {  // wc: R (9) and t (3)
+      const int q = tid - 64;
+      if (q < 9) P->wc.r[q] = (float)tN[(q / 3) * 4 + q % 3];
+      else P->wc.t[q - 9] = (float)tN[(q - 9) * 4 + 3];
+    } else if (tid >= 128 && tid < 140) {  // cw = rigid inverse: R^T and -R^T t
+      const int q = tid - 128;
+      if (q < 9) P->cw.r[q] = (float)tN[(q % 3) * 4 + q / 3];
+      else P->cw.t[q - 9] = (float)rigid_inverse_t(tN, q - 9);
+    } else if (tid == 192) {
+      a.ctrl->executed = a.ctrl->executed + 1;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) __hip_atomic_store(&a.ctrl->arrive, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void icp_begin(const kf_pose* guess, kf_pose* work, kf_icp_ctrl* ctrl,

@@ -20,11 +20,16 @@
 
 namespace kf {
 
+// Row r of -R^T t for the rigid transform T (row-major 4x4).
+KF_HD double rigid_inverse_t(const double* T, int r) {
+  return -((T[0 * 4 + r] * T[3] + T[1 * 4 + r] * T[7]) + T[2 * 4 + r] * T[11]);
+}
+
 // Ti = inverse of the rigid transform T (row-major 4x4).
 KF_HD void rigid_inverse(const double* T, double* Ti) {
   for (int r = 0; r < 3; ++r) {
     for (int c = 0; c < 3; ++c) Ti[r * 4 + c] = T[c * 4 + r];
-    Ti[r * 4 + 3] = -((T[0 * 4 + r] * T[3] + T[1 * 4 + r] * T[7]) + T[2 * 4 + r] * T[11]);
+    Ti[r * 4 + 3] = rigid_inverse_t(T, r);
   }
   Ti[12] = 0.0;
   Ti[13] = 0.0;
@@ -32,13 +37,16 @@ KF_HD void rigid_inverse(const double* T, double* Ti) {
   Ti[15] = 1.0;
 }
 
+// Entry (i, j) of A * B (row-major 4x4).
+KF_HD double mat4_mul_entry(const double* A, const double* B, int i, int j) {
+  return ((A[i * 4 + 0] * B[0 * 4 + j] + A[i * 4 + 1] * B[1 * 4 + j]) + A[i * 4 + 2] * B[2 * 4 + j]) +
+         A[i * 4 + 3] * B[3 * 4 + j];
+}
+
 // C = A * B (row-major 4x4); C must not alias A or B.
 KF_HD void mat4_mul(const double* A, const double* B, double* C) {
   for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j)
-      C[i * 4 + j] = ((A[i * 4 + 0] * B[0 * 4 + j] + A[i * 4 + 1] * B[1 * 4 + j]) +
-                      A[i * 4 + 2] * B[2 * 4 + j]) +
-                     A[i * 4 + 3] * B[3 * 4 + j];
+    for (int j = 0; j < 4; ++j) C[i * 4 + j] = mat4_mul_entry(A, B, i, j);
 }
 
 // One Newton step of the polar decomposition on the rotation block,
@@ -76,24 +84,29 @@ KF_HD void pose_finish_fields(kf_pose* p) {
 
 // Euler-ZYX increment R = Rz(g) Ry(b) Rx(a) with translation t (same element
 // formulas as the reference's incremental transform).
+KF_HD double euler_zyx_entry(int e, double ca, double sa, double cb, double sb, double cg, double sg, double tx,
+                             double ty, double tz) {
+  switch (e) {
+    case 0: return cg * cb;
+    case 1: return -sg * ca + (cg * sb) * sa;
+    case 2: return sg * sa + (cg * sb) * ca;
+    case 3: return tx;
+    case 4: return sg * cb;
+    case 5: return cg * ca + (sg * sb) * sa;
+    case 6: return -cg * sa + (sg * sb) * ca;
+    case 7: return ty;
+    case 8: return -sb;
+    case 9: return cb * sa;
+    case 10: return cb * ca;
+    case 11: return tz;
+    case 15: return 1.0;
+    default: return 0.0;
+  }
+}
+
 KF_HD void euler_zyx(double a, double b, double g, double tx, double ty, double tz, double* D) {
   const double ca = cos(a), sa = sin(a), cb = cos(b), sb = sin(b), cg = cos(g), sg = sin(g);
-  D[0] = cg * cb;
-  D[1] = -sg * ca + (cg * sb) * sa;
-  D[2] = sg * sa + (cg * sb) * ca;
-  D[3] = tx;
-  D[4] = sg * cb;
-  D[5] = cg * ca + (sg * sb) * sa;
-  D[6] = -cg * sa + (sg * sb) * ca;
-  D[7] = ty;
-  D[8] = -sb;
-  D[9] = cb * sa;
-  D[10] = cb * ca;
-  D[11] = tz;
-  D[12] = 0.0;
-  D[13] = 0.0;
-  D[14] = 0.0;
-  D[15] = 1.0;
+  for (int e = 0; e < 16; ++e) D[e] = euler_zyx_entry(e, ca, sa, cb, sb, cg, sg, tx, ty, tz);
 }
 
 // Solves the 6x6 SPD system A x = g (A given by its 21 upper-triangle entries,
