@@ -1,4 +1,4 @@
-// kf_icp.hip — projective point-to-plane ICP on gfx950, one launch per iteration.
+// kf_icp.hip — projective point-to-plane ICP on gfx950, two launches per iteration.
 //
 // Replaces d_find_corresponding / ICP::FindCorresponding (Source/src/ICP.cu:116-262),
 // the host block sum, determinant guard and Eigen LLT of ICP::Iterate_ICP
@@ -15,13 +15,13 @@
 //
 // Reduction: every thread accumulates the 28 FP32 terms (21 A^TA, 6 A^Tb, b^2)
 // over a grid-stride run of pixels; waves combine with DPP lane moves (quad
-// permutes, row rotations, row broadcasts) and blocks through LDS.  In compensated mode each term is carried as a
-// (sum, error) pair updated with the error-free TwoSum, so a block partial is
-// accurate to ~2^-48 relative; plain mode uses ordinary FP32 sums.  Blocks write
-// their partials, and the last block to arrive (device-scope counter) adds them
-// in FP64 in block order, solves by FP64 Cholesky, composes the Euler-ZYX
-// increment onto the FP64 pose and refreshes the FP32 pose copies, so the next
-// iteration's launch reads the new pose without any host round trip.
+// permutes, row rotations, row broadcasts) and blocks through LDS.  In
+// compensated mode each term is carried as a (sum, error) pair updated with the
+// error-free TwoSum, so a block partial is accurate to ~2^-48 relative; plain mode uses ordinary FP32 sums.  Blocks write
+// their partials; a one-block launch adds them in FP64 in block order, solves by
+// FP64 Cholesky, composes the Euler-ZYX increment onto the FP64 pose and refreshes
+// the FP32 pose copies, so the next iteration's launch reads the new pose without
+// any host round trip.
 #include "kf_device.hpp"
 #include "kf_internal.hpp"
 #include "kf_pose_math.hpp"
@@ -109,13 +109,12 @@ __device__ __forceinline__ void dpp_step(float& s, float& c, int = 0) {
   }
 }
 
+// Pass 1 of an iteration: association and the block partials (one slot per block).
 template <bool COMP>
-__global__ __launch_bounds__(kThreads) void icp_iteration(IcpArgs a) {
+__global__ __launch_bounds__(kThreads) void icp_partials(IcpArgs a) {
   __shared__ float red_s[4][kTerms];
   __shared__ float red_c[4][kTerms];
   __shared__ int red_n[4][2];
-  __shared__ double fin[16][kTerms + 2];
-  __shared__ int is_last;
   if (a.mode == 0 && a.ctrl->stop) return;  // uniform over the grid
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -196,19 +195,16 @@ __global__ __launch_bounds__(kThreads) void icp_iteration(IcpArgs a) {
     slot[56] = __int_as_float(ni);
     slot[57] = __int_as_float(nc);
   }
+}
 
-  // Last-block-done: publish the partial, count arrivals at device scope.
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) {
-    const unsigned prev = __hip_atomic_fetch_add(&a.ctrl->arrive, 1u, __ATOMIC_ACQ_REL,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-
+// Pass 2 of an iteration (one block): FP64 sum of the block partials in block
+// order, then the solve and the pose update (or, in reduce mode, the sums).  A
+// separate launch makes the partials visible through the kernel boundary instead
+// of a device-scope fence per block (each fence writes back and invalidates the L2).
+__global__ __launch_bounds__(kThreads) void icp_solve(IcpArgs a, int nb) {
+  __shared__ double fin[16][kTerms + 2];
+  if (a.mode == 0 && a.ctrl->stop) return;  // uniform
+  const int tid = threadIdx.x;
   // FP64 cross-block sum.  Thread (g, j) reads float4 j of the slots of blocks
   // g, g + 16, ... (j < 14: terms 2j and 2j + 1 as (hi, lo) pairs; j = 14: the
   // inlier and candidate counts) with kFinUnroll independent loads in flight,
@@ -218,7 +214,6 @@ __global__ __launch_bounds__(kThreads) void icp_iteration(IcpArgs a) {
     double a0 = 0.0, a1 = 0.0;
     if (j < 15) {
       const float4* base = reinterpret_cast<const float4*>(a.partials) + j;
-      const int nb = (int)gridDim.x;
       for (int b0 = g; b0 < nb; b0 += 16 * kFinUnroll) {
         float4 v[kFinUnroll];
 #pragma unroll
@@ -254,8 +249,6 @@ __global__ __launch_bounds__(kThreads) void icp_iteration(IcpArgs a) {
   const double* sum = fin[0];
   if (a.mode == 1) {
     if (tid < kTerms + 2) a.sums_out[tid] = sum[tid];
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(&a.ctrl->arrive, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
   __shared__ double tx[6], tcs[6], tD[16], tT[16], tN[16];
@@ -310,9 +303,8 @@ __global__ __launch_bounds__(kThreads) void icp_iteration(IcpArgs a) {
       a.ctrl->executed = a.ctrl->executed + 1;
     }
   }
-  __syncthreads();
-  if (tid == 0) __hip_atomic_store(&a.ctrl->arrive, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
+
 
 __global__ void icp_begin(const kf_pose* guess, kf_pose* work, kf_icp_ctrl* ctrl,
                           const kf_icp_stats* user, kf_icp_stats* st) {
@@ -417,10 +409,12 @@ kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_fr
     const int blocks = level_blocks(t, a.W, a.H, l);
     for (int it = 0; it < p.iterations[l]; ++it) {
       if (p.compensated)
-        hipLaunchKernelGGL(icp_iteration<true>, dim3(blocks), dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL(icp_partials<true>, dim3(blocks), dim3(kThreads), 0, s, a);
       else
-        hipLaunchKernelGGL(icp_iteration<false>, dim3(blocks), dim3(kThreads), 0, s, a);
-      KF_TRY(check_launch("icp_iteration"));
+        hipLaunchKernelGGL(icp_partials<false>, dim3(blocks), dim3(kThreads), 0, s, a);
+      KF_TRY(check_launch("icp_partials"));
+      hipLaunchKernelGGL(icp_solve, dim3(1), dim3(kThreads), 0, s, a, blocks);
+      KF_TRY(check_launch("icp_solve"));
     }
   }
   hipLaunchKernelGGL(icp_end, dim3(1), dim3(64), 0, s, (const kf_pose*)t->d_work, d_out,
@@ -442,10 +436,12 @@ kf_status launch_icp_reduce(kf_tracker* t, int level, const kf_frame_maps& live,
   a.mode = 1;
   const int blocks = level_blocks(t, a.W, a.H, level);
   if (t->params.compensated)
-    hipLaunchKernelGGL(icp_iteration<true>, dim3(blocks), dim3(kThreads), 0, s, a);
+    hipLaunchKernelGGL(icp_partials<true>, dim3(blocks), dim3(kThreads), 0, s, a);
   else
-    hipLaunchKernelGGL(icp_iteration<false>, dim3(blocks), dim3(kThreads), 0, s, a);
-  return check_launch("icp_iteration(reduce)");
+    hipLaunchKernelGGL(icp_partials<false>, dim3(blocks), dim3(kThreads), 0, s, a);
+  KF_TRY(check_launch("icp_partials(reduce)"));
+  hipLaunchKernelGGL(icp_solve, dim3(1), dim3(kThreads), 0, s, a, blocks);
+  return check_launch("icp_solve(reduce)");
 }
 
 kf_status launch_pose_constant_velocity(const kf_pose* d_prev2, const kf_pose* d_prev1,

@@ -59,7 +59,7 @@ struct kf_volume {
 
 // ICP scratch and per-iteration control block (device memory).
 struct kf_icp_ctrl {
-  unsigned int arrive;  // blocks finished in the current iteration (reset by the last block)
+  unsigned int reserved;
   int stop;             // set when a system was singular: later iterations skip
   int executed;         // iterations that produced an update
   int pad;
