@@ -30,11 +30,14 @@ namespace kf {
 namespace {
 
 constexpr int kTerms = 28;   // 21 + 6 + 1 float terms
+constexpr int kPad = 32;     // terms padded to a power of two for the wave reduction
 constexpr int kSlot = 64;    // floats per block partial: 28 (hi,lo) + 2 int counts
 constexpr int kThreads = 256;
 constexpr double kPivotRelEps = 1e-9;
 constexpr int kMinInliers = 6;
-constexpr int kFinUnroll = 12;  // partial-slot loads in flight per thread in the final sum
+constexpr int kFinUnroll = 4;       // partial-slot loads in flight per thread in the final sum
+constexpr int kSolveThreads = 1024;  // one block: 64 groups x 16 float4 columns of the slots
+constexpr int kSolveGroups = kSolveThreads / 16;
 
 struct IcpArgs {
   const float *lvx, *lvy, *lvz, *lnx, *lny, *lnz;  // live level maps (camera frame)
@@ -60,29 +63,52 @@ __device__ __forceinline__ void two_sum(float& s, float& c, float v) {
   c = c + e;
 }
 
-// Evaluates the association of model pixel i.  Returns 0 = not a candidate,
-// 1 = candidate but rejected, 2 = inlier (A, b filled).
-__device__ __forceinline__ int associate(const IcpArgs& a, const kf_posef& wc, const kf_posef& cw,
-                                         int i, float* A, float& b) {
-  const float lz_here = a.lvz[i];
-  const f3 s = f3{a.mvx[i], a.mvy[i], a.mvz[i]};
-  const f3 m = f3{a.mnx[i], a.mny[i], a.mnz[i]};
-  if (!(lz_here == lz_here) || !finite3(s) || !finite3(m)) return 0;
-  const f3 sc = xform(cw, s);
-  if (!(sc.z > 1e-6f)) return 1;
+// Association of model pixel i in three phases, so that a thread can issue the
+// loads of two pixels before using either: model loads, projection (live pixel
+// j), live loads, gating and the row.  Returns 0 = not a candidate, 1 = candidate
+// but rejected, 2 = inlier (A, b filled).
+struct PxModel {
+  f3 s, m;
+  float lz_here;
+};
+struct PxLive {
+  f3 dc, nc;
+};
+
+__device__ __forceinline__ PxModel load_model(const IcpArgs& a, int i) {
+  return PxModel{f3{a.mvx[i], a.mvy[i], a.mvz[i]}, f3{a.mnx[i], a.mny[i], a.mnz[i]}, a.lvz[i]};
+}
+
+// Live pixel the model point projects to, or -1 (0 = not a candidate, 1 = rejected
+// in r when j < 0).
+__device__ __forceinline__ int project_model(const IcpArgs& a, const kf_posef& cw, const PxModel& q, int& r) {
+  r = 0;
+  if (!(q.lz_here == q.lz_here) || !finite3(q.s) || !finite3(q.m)) return -1;
+  r = 1;
+  const f3 sc = xform(cw, q.s);
+  if (!(sc.z > 1e-6f)) return -1;
   const float iz = 1.0f / sc.z;
   const float u = (a.fx * sc.x + a.cx * sc.z) * iz;
   const float v = (a.fy * sc.y + a.cy * sc.z) * iz;
   const float ru = round_half_up(u), rv = round_half_up(v);
-  if (!(ru >= 0.0f && ru < (float)a.W && rv >= 0.0f && rv < (float)a.H)) return 1;
-  const int j = (int)rv * a.W + (int)ru;
-  const f3 dc = f3{a.lvx[j], a.lvy[j], a.lvz[j]};
-  const f3 nc = f3{a.lnx[j], a.lny[j], a.lnz[j]};
-  if (!finite3(dc) || !finite3(nc)) return 1;
-  const f3 d = xform(wc, dc);
-  const f3 n = rot(wc.r, nc);
-  const f3 df = sub(d, s);
+  if (!(ru >= 0.0f && ru < (float)a.W && rv >= 0.0f && rv < (float)a.H)) return -1;
+  return (int)rv * a.W + (int)ru;
+}
+
+__device__ __forceinline__ PxLive load_live(const IcpArgs& a, int j) {
+  const int jj = j < 0 ? 0 : j;  // always in bounds; the result is unused when j < 0
+  return PxLive{f3{a.lvx[jj], a.lvy[jj], a.lvz[jj]}, f3{a.lnx[jj], a.lny[jj], a.lnz[jj]}};
+}
+
+__device__ __forceinline__ int finish_row(const IcpArgs& a, const kf_posef& wc, const PxModel& q, const PxLive& l,
+                                          int j, int r, float* A, float& b) {
+  if (j < 0) return r;
+  if (!finite3(l.dc) || !finite3(l.nc)) return 1;
+  const f3 d = xform(wc, l.dc);
+  const f3 n = rot(wc.r, l.nc);
+  const f3 df = sub(d, q.s);
   if (!(sqrtf(dot(df, df)) < a.dist_thr)) return 1;
+  const f3 m = q.m, s = q.s;
   const float cs = dot(n, m);
   if (!(cs > a.cos_thr && cs < 1.0f + 1e-6f)) return 1;
   A[0] = m.y * d.z - m.z * d.y;
@@ -95,12 +121,31 @@ __device__ __forceinline__ int associate(const IcpArgs& a, const kf_posef& wc, c
   return 2;
 }
 
-// One DPP step of the wave reduction on a (sum, error) pair.
-template <int CTRL, int ROW_MASK, bool COMP>
-__device__ __forceinline__ void dpp_step(float& s, float& c, int = 0) {
-  const float so = dpp_f<CTRL, ROW_MASK>(s);
+template <bool COMP>
+__device__ __forceinline__ void add_term(float& s, float& c, float v) {
+  if (COMP) two_sum(s, c, v); else s = s + v;
+}
+
+// Adds the 28 terms of one inlier row: 21 upper-triangle A^T A, 6 A^T b, b^2.
+template <bool COMP>
+__device__ __forceinline__ void accumulate(float* s, float* c, const float* A, float b) {
+  int k = 0;
+#pragma unroll
+  for (int p = 0; p < 6; ++p)
+#pragma unroll
+    for (int q = p; q < 6; ++q) {
+      add_term<COMP>(s[k], c[k], A[p] * A[q]);
+      ++k;
+    }
+#pragma unroll
+  for (int p = 0; p < 6; ++p) add_term<COMP>(s[21 + p], c[21 + p], A[p] * b);
+  add_term<COMP>(s[27], c[27], b * b);
+}
+
+// Combines (s, c) with a partner's (so, co): TwoSum of the sums, errors added.
+template <bool COMP>
+__device__ __forceinline__ void combine(float& s, float& c, float so, float co) {
   if (COMP) {
-    const float co = dpp_f<CTRL, ROW_MASK>(c);
     float e = 0.0f;
     two_sum(s, e, so);
     c = (c + co) + e;
@@ -109,68 +154,108 @@ __device__ __forceinline__ void dpp_step(float& s, float& c, int = 0) {
   }
 }
 
+// One recursive-halving step of the wave reduction (reduce-scatter): lanes paired
+// by the DPP pattern CTRL hold the same H terms; the lane with `upper` keeps the
+// upper H/2, its partner the lower H/2, and each adds the partner's copy of the
+// half it keeps.
+template <int CTRL, int H, bool COMP>
+__device__ __forceinline__ void halve(float* s, float* c, bool upper) {
+#pragma unroll
+  for (int k = 0; k < H / 2; ++k) {
+    const float ks = upper ? s[k + H / 2] : s[k];
+    const float kc = upper ? c[k + H / 2] : c[k];
+    const float gs = upper ? s[k] : s[k + H / 2];
+    const float gc = upper ? c[k] : c[k + H / 2];
+    const float rs = dpp_f<CTRL>(gs);
+    const float rc = COMP ? dpp_f<CTRL>(gc) : 0.0f;
+    float ns = ks, nc = kc;
+    combine<COMP>(ns, nc, rs, rc);
+    s[k] = ns;
+    c[k] = nc;
+  }
+}
+
 // Pass 1 of an iteration: association and the block partials (one slot per block).
 template <bool COMP>
 __global__ __launch_bounds__(kThreads) void icp_partials(IcpArgs a) {
-  __shared__ float red_s[4][kTerms];
-  __shared__ float red_c[4][kTerms];
+  __shared__ float red_s[4][kPad];
+  __shared__ float red_c[4][kPad];
   __shared__ int red_n[4][2];
   if (a.mode == 0 && a.ctrl->stop) return;  // uniform over the grid
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const kf_posef wc = a.pose->wc, cw = a.pose->cw;
-  float s[kTerms], c[kTerms];
+  float s[kPad], c[kPad];  // terms 28..31 stay zero (padding for the halving)
 #pragma unroll
-  for (int k = 0; k < kTerms; ++k) {
+  for (int k = 0; k < kPad; ++k) {
     s[k] = 0.0f;
     c[k] = 0.0f;
   }
   int n_in = 0, n_cand = 0;
   const int P = a.W * a.H;
-  for (int i = blockIdx.x * kThreads + tid; i < P; i += gridDim.x * kThreads) {
-    float A[6], b;
-    const int r = associate(a, wc, cw, i, A, b);
-    if (a.mask) a.mask[i] = (uint8_t)(r == 2);
-    if (r == 0) continue;
-    ++n_cand;
-    if (r != 2) continue;
-    ++n_in;
-    int k = 0;
-#pragma unroll
-    for (int p = 0; p < 6; ++p)
-#pragma unroll
-      for (int q = p; q < 6; ++q) {
-        const float v = A[p] * A[q];
-        if (COMP) two_sum(s[k], c[k], v); else s[k] = s[k] + v;
-        ++k;
-      }
-#pragma unroll
-    for (int p = 0; p < 6; ++p) {
-      const float v = A[p] * b;
-      if (COMP) two_sum(s[21 + p], c[21 + p], v); else s[21 + p] = s[21 + p] + v;
+  const int stride = gridDim.x * kThreads;
+  // Two pixels per step (i0, i0 + stride): both pixels' loads are in flight together.
+  for (int i0 = blockIdx.x * kThreads + tid; i0 < P; i0 += 2 * stride) {
+    const int i1 = i0 + stride;
+    const bool has1 = i1 < P;
+    const PxModel q0 = load_model(a, i0);
+    const PxModel q1 = load_model(a, has1 ? i1 : i0);
+    int r0, r1;
+    const int j0 = project_model(a, cw, q0, r0);
+    int j1 = project_model(a, cw, q1, r1);
+    if (!has1) {
+      j1 = -1;
+      r1 = 0;
     }
-    if (COMP) two_sum(s[27], c[27], b * b); else s[27] = s[27] + b * b;
+    const PxLive l0 = load_live(a, j0);
+    const PxLive l1 = load_live(a, j1);
+    float A0[6], b0, A1[6], b1;
+    r0 = finish_row(a, wc, q0, l0, j0, r0, A0, b0);
+    r1 = finish_row(a, wc, q1, l1, j1, r1, A1, b1);
+    if (a.mask) {
+      a.mask[i0] = (uint8_t)(r0 == 2);
+      if (has1) a.mask[i1] = (uint8_t)(r1 == 2);
+    }
+    n_cand += (r0 != 0 ? 1 : 0) + (r1 != 0 ? 1 : 0);
+    if (r0 == 2) {
+      ++n_in;
+      accumulate<COMP>(s, c, A0, b0);
+    }
+    if (r1 == 2) {
+      ++n_in;
+      accumulate<COMP>(s, c, A1, b1);
+    }
   }
 
-  // Wave reduction with DPP lane moves (no LDS round trips): the (sum, error)
-  // pairs of all 64 lanes end up in lane 63.
-#pragma unroll
-  for (int k = 0; k < kTerms; ++k) {
-    dpp_step<0xB1, 0xf, COMP>(s[k], c[k]);
-    dpp_step<0x4E, 0xf, COMP>(s[k], c[k]);
-    dpp_step<0x124, 0xf, COMP>(s[k], c[k]);
-    dpp_step<0x128, 0xf, COMP>(s[k], c[k]);
-    dpp_step<0x142, 0xA, COMP>(s[k], c[k]);
-    dpp_step<0x143, 0xC, COMP>(s[k], c[k]);
+  // Wave reduction by recursive halving (reduce-scatter): five pairings, each
+  // halving the terms a lane holds — row mirror (lane bit 3), half-row mirror
+  // (bit 2), quad xor 2 (bit 1), quad xor 1 (bit 0), xor 16 (bit 4) — then xor 32
+  // adds the two half-waves.  Lane l < 32 ends with the full wave sum of term
+  // 16*b3 + 8*b2 + 4*b1 + 2*b0 + b4 (b = bits of l).  About 1/5 of the work of
+  // reducing all 28 terms down the whole butterfly.
+  halve<0x140, 32, COMP>(s, c, (lane & 8) != 0);
+  halve<0x141, 16, COMP>(s, c, (lane & 4) != 0);
+  halve<0x4E, 8, COMP>(s, c, (lane & 2) != 0);
+  halve<0xB1, 4, COMP>(s, c, (lane & 1) != 0);
+  {
+    const bool up = (lane & 16) != 0;
+    const float gs = up ? s[0] : s[1], gc = up ? c[0] : c[1];
+    float ks = up ? s[1] : s[0], kc = up ? c[1] : c[0];
+    const float rs = __shfl_xor(gs, 16, kWave);
+    const float rc = COMP ? __shfl_xor(gc, 16, kWave) : 0.0f;
+    combine<COMP>(ks, kc, rs, rc);
+    const float ps = __shfl_xor(ks, 32, kWave);
+    const float pc = COMP ? __shfl_xor(kc, 32, kWave) : 0.0f;
+    combine<COMP>(ks, kc, ps, pc);
+    const int term = ((lane & 8) << 1) | ((lane & 4) << 1) | ((lane & 2) << 1) | ((lane & 1) << 1) | ((lane >> 4) & 1);
+    if (lane < 32) {
+      red_s[wave][term] = ks;
+      red_c[wave][term] = kc;
+    }
   }
   n_in = wave_sum_i_lane63(n_in);
   n_cand = wave_sum_i_lane63(n_cand);
   if (lane == 63) {
-#pragma unroll
-    for (int k = 0; k < kTerms; ++k) {
-      red_s[wave][k] = s[k];
-      red_c[wave][k] = c[k];
-    }
     red_n[wave][0] = n_in;
     red_n[wave][1] = n_cand;
   }
@@ -201,24 +286,24 @@ __global__ __launch_bounds__(kThreads) void icp_partials(IcpArgs a) {
 // order, then the solve and the pose update (or, in reduce mode, the sums).  A
 // separate launch makes the partials visible through the kernel boundary instead
 // of a device-scope fence per block (each fence writes back and invalidates the L2).
-__global__ __launch_bounds__(kThreads) void icp_solve(IcpArgs a, int nb) {
-  __shared__ double fin[16][kTerms + 2];
+__global__ __launch_bounds__(kSolveThreads) void icp_solve(IcpArgs a, int nb) {
+  __shared__ double fin[kSolveGroups][kTerms + 2];
   if (a.mode == 0 && a.ctrl->stop) return;  // uniform
   const int tid = threadIdx.x;
   // FP64 cross-block sum.  Thread (g, j) reads float4 j of the slots of blocks
-  // g, g + 16, ... (j < 14: terms 2j and 2j + 1 as (hi, lo) pairs; j = 14: the
+  // g, g + 64, ... (j < 14: terms 2j and 2j + 1 as (hi, lo) pairs; j = 14: the
   // inlier and candidate counts) with kFinUnroll independent loads in flight,
-  // then the 16 group totals of each term are added in group order.
+  // then the 64 group totals of each term are added in group order.
   {
     const int j = tid & 15, g = tid >> 4;
     double a0 = 0.0, a1 = 0.0;
     if (j < 15) {
       const float4* base = reinterpret_cast<const float4*>(a.partials) + j;
-      for (int b0 = g; b0 < nb; b0 += 16 * kFinUnroll) {
+      for (int b0 = g; b0 < nb; b0 += kSolveGroups * kFinUnroll) {
         float4 v[kFinUnroll];
 #pragma unroll
         for (int u = 0; u < kFinUnroll; ++u) {
-          const int bb = b0 + 16 * u;
+          const int bb = b0 + kSolveGroups * u;
           v[u] = bb < nb ? base[(size_t)bb * (kSlot / 4)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
 #pragma unroll
@@ -239,7 +324,7 @@ __global__ __launch_bounds__(kThreads) void icp_solve(IcpArgs a, int nb) {
   __syncthreads();
   if (tid < kTerms + 2) {
     double t = fin[0][tid];
-    for (int gg = 1; gg < 16; ++gg) t = t + fin[gg][tid];
+    for (int gg = 1; gg < kSolveGroups; ++gg) t = t + fin[gg][tid];
     fin[0][tid] = t;
   }
   __syncthreads();
@@ -413,7 +498,7 @@ kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_fr
       else
         hipLaunchKernelGGL(icp_partials<false>, dim3(blocks), dim3(kThreads), 0, s, a);
       KF_TRY(check_launch("icp_partials"));
-      hipLaunchKernelGGL(icp_solve, dim3(1), dim3(kThreads), 0, s, a, blocks);
+      hipLaunchKernelGGL(icp_solve, dim3(1), dim3(kSolveThreads), 0, s, a, blocks);
       KF_TRY(check_launch("icp_solve"));
     }
   }
@@ -440,7 +525,7 @@ kf_status launch_icp_reduce(kf_tracker* t, int level, const kf_frame_maps& live,
   else
     hipLaunchKernelGGL(icp_partials<false>, dim3(blocks), dim3(kThreads), 0, s, a);
   KF_TRY(check_launch("icp_partials(reduce)"));
-  hipLaunchKernelGGL(icp_solve, dim3(1), dim3(kThreads), 0, s, a, blocks);
+  hipLaunchKernelGGL(icp_solve, dim3(1), dim3(kSolveThreads), 0, s, a, blocks);
   return check_launch("icp_solve(reduce)");
 }
 

@@ -11,10 +11,10 @@
 //      z-interval outside which no voxel can pass the exact per-voxel test (the
 //      column is a line in camera space, the frustum is convex).
 //   2. integrate: one wavefront per column (waves stride over the columns) walks
-//      its span with 16-byte vector accesses (2 FP32 or 4 FP16 voxels per lane),
-//      evaluates the exact FP32 test per voxel (same expressions as the
-//      reference's per-voxel body, in metres and with a z-forward camera), and
-//      loads/stores a vector only if one of its voxels is updated.
+//      its span 64 voxels at a time (one voxel per lane, coalesced), evaluates the
+//      exact FP32 test per voxel (same expressions as the reference's per-voxel
+//      body, in metres and with a z-forward camera), loads a voxel only if it is
+//      updated and stores it only if its bits change.
 #include "kf_device.hpp"
 #include "kf_internal.hpp"
 
@@ -135,7 +135,6 @@ __device__ __forceinline__ bool voxel_sample(const SweepArgs& a, float ax, float
 
 template <int FMT>
 __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
-  constexpr int V = (FMT == KF_VOXEL_F16) ? 4 : 2;  // voxels per 16-byte lane access
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
   const kf_posef P = a.pose->cw;
   const int ncol = a.n * a.n;
@@ -153,50 +152,54 @@ __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
     const float az = (P.r[6] * wx + P.r[7] * wy) + P.t[2];
     const size_t col = (size_t)c * (size_t)a.n;
     unsigned long long occ_bits = 0ull;  // z-bricks receiving f != 1 from this lane
-    for (int zb = (z0 & ~(V - 1)) + lane * V; zb <= z1; zb += kWave * V) {
-      float fs[V];
-      unsigned upd = 0;
-#pragma unroll
-      for (int j = 0; j < V; ++j) {
-        fs[j] = 0.0f;
-        if (voxel_sample(a, ax, ay, az, P.r, zb + j, fs[j])) upd |= 1u << j;
+    // One voxel per lane per step: a wave covers 64 consecutive z (256 B of FP16
+    // or 512 B of FP32 voxels, fully coalesced), so few lanes idle at span ends.
+    for (int zb = z0; zb <= z1; zb += kWave) {
+      const int z = zb + lane;
+      float fs = 0.0f;
+      const bool upd = z <= z1 && voxel_sample(a, ax, ay, az, P.r, z, fs);
+      if (upd) {
+        ++count;
+        const size_t idx = col + (size_t)z;
+        if (FMT == KF_VOXEL_F16) {
+          uint32_t* p = static_cast<uint32_t*>(a.vox) + idx;
+          const uint32_t w = *p;
+          const float fo = half_bits_to_float(w & 0xffffu);
+          const float wo = (float)(w >> 16);
+          // Free space (f_old = f = 1) averages to exactly 1: (w + 1) / (w + 1).
+          const float fn = (fo == 1.0f && fs == 1.0f) ? 1.0f : (fo * wo + fs) / (wo + 1.0f);
+          const float wn = fminf(wo + 1.0f, a.maxw);
+          const uint32_t nw = float_to_half_bits(fn) | ((uint32_t)wn << 16);
+          if (nw != w) *p = nw;  // saturated free space (f = 1, w = max) is left as it is
+          if (a.occ && (nw & 0xffffu) != 0x3C00u) occ_bits |= occ_zbits(z);
+        } else {
+          float2* p = static_cast<float2*>(a.vox) + idx;
+          const float2 v = *p;
+          const float fn = (v.x == 1.0f && fs == 1.0f) ? 1.0f : (v.x * v.y + fs) / (v.y + 1.0f);
+          const float wn = fminf(v.y + 1.0f, a.maxw);
+          if (__float_as_uint(fn) != __float_as_uint(v.x) || __float_as_uint(wn) != __float_as_uint(v.y))
+            *p = make_float2(fn, wn);
+          if (a.occ && fn != 1.0f) occ_bits |= occ_zbits(z);
+        }
       }
-      if (!upd) continue;
-      count += __builtin_popcount(upd);
-      const size_t idx = col + (size_t)zb;
-      if (FMT == KF_VOXEL_F16) {
-        uint4* p = reinterpret_cast<uint4*>(static_cast<uint32_t*>(a.vox) + idx);
-        uint4 w4 = *p;
-        uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (upd & (1u << j)) {
-            const float fo = half_bits_to_float(w[j] & 0xffffu);
-            const float wo = (float)(w[j] >> 16);
-            // Free space (f_old = f = 1) averages to exactly 1: (w + 1) / (w + 1).
-            const float fn = (fo == 1.0f && fs[j] == 1.0f) ? 1.0f : (fo * wo + fs[j]) / (wo + 1.0f);
-            const float wn = fminf(wo + 1.0f, a.maxw);
-            w[j] = float_to_half_bits(fn) | ((uint32_t)wn << 16);
-            if (a.occ && (w[j] & 0xffffu) != 0x3C00u) occ_bits |= occ_zbits(zb + j);
+      if (a.mask) {  // 64 consecutive voxels -> up to three 32-bit mask words
+        const unsigned long long m = __ballot(upd);
+        if (m) {
+          const size_t base = col + (size_t)zb;
+          const int sh = (int)(base & 31);
+          const size_t w0 = base >> 5;
+          if (lane == 0) {
+            const uint32_t v0 = (uint32_t)(m << sh);
+            if (v0) atomicOr(&a.mask[w0], v0);
+          } else if (lane == 1) {
+            const uint32_t v1 = (uint32_t)(m >> (32 - sh));
+            if (v1) atomicOr(&a.mask[w0 + 1], v1);
+          } else if (lane == 2 && sh > 0) {
+            const uint32_t v2 = (uint32_t)(m >> (64 - sh));
+            if (v2) atomicOr(&a.mask[w0 + 2], v2);
           }
         }
-        *p = make_uint4(w[0], w[1], w[2], w[3]);
-      } else {
-        float4* p = reinterpret_cast<float4*>(static_cast<float2*>(a.vox) + idx);
-        float4 v4 = *p;
-        float fw[4] = {v4.x, v4.y, v4.z, v4.w};
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          if (upd & (1u << j)) {
-            const float fo = fw[2 * j], wo = fw[2 * j + 1];
-            fw[2 * j] = (fo == 1.0f && fs[j] == 1.0f) ? 1.0f : (fo * wo + fs[j]) / (wo + 1.0f);
-            fw[2 * j + 1] = fminf(wo + 1.0f, a.maxw);
-            if (a.occ && fw[2 * j] != 1.0f) occ_bits |= occ_zbits(zb + j);
-          }
-        }
-        *p = make_float4(fw[0], fw[1], fw[2], fw[3]);
       }
-      if (a.mask) atomicOr(&a.mask[idx >> 5], upd << (idx & 31));
     }
     if (a.occ) occ_mark_column(a.occ, a.occ_nb, ix, iy, occ_bits, lane);
   }
