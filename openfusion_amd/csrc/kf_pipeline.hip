@@ -149,23 +149,27 @@ __global__ void prior_guess(PriorState* st, kf_pose* poses) {
 // Accept the warm-started ICP or arm the plain ICP (its skip flag).
 __global__ void prior_decide(PriorState* st, const kf_icp_stats* sp, kf_icp_stats* plain, kf_pose* poses,
                              float ratio, float min_frac) {
-  if (threadIdx.x != 0) return;
+  // Running median of the window: lane i ranks value i (ties broken by index), the
+  // lanes of the middle rank(s) publish their values — one pass instead of a serial
+  // sort.  Launched with 64 threads; kWindow <= 64.
+  __shared__ double v[kWindow], mid[2];
+  const int i = threadIdx.x;
+  const int n = st->win_count;
+  if (i < n) v[i] = st->window[i];
+  __syncthreads();
+  if (i < n) {
+    const double x = v[i];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) rank += (v[j] < x || (v[j] == x && j < i)) ? 1 : 0;
+    if (rank == n / 2) mid[1] = x;
+    if ((n & 1) == 0 && rank == n / 2 - 1) mid[0] = x;
+  }
+  __syncthreads();
+  if (i != 0) return;
   bool fail = sp->status != 0 || sp->iterations == 0 || !(sp->residual_rms == sp->residual_rms);
   if (!fail && sp->candidates > 0 && (double)sp->inliers < (double)min_frac * (double)sp->candidates) fail = true;
-  if (!fail && st->win_count >= kWindowMin) {
-    double v[kWindow];
-    const int n = st->win_count;
-    for (int i = 0; i < n; ++i) v[i] = st->window[i];
-    for (int i = 1; i < n; ++i) {
-      const double x = v[i];
-      int j = i - 1;
-      while (j >= 0 && v[j] > x) {
-        v[j + 1] = v[j];
-        --j;
-      }
-      v[j + 1] = x;
-    }
-    const double med = (n & 1) ? v[n / 2] : 0.5 * (v[n / 2 - 1] + v[n / 2]);
+  if (!fail && n >= kWindowMin) {
+    const double med = (n & 1) ? mid[1] : 0.5 * (mid[0] + mid[1]);
     if (sp->residual_rms > (double)ratio * med) fail = true;
   }
   st->used_prior = 1;

@@ -90,6 +90,8 @@ struct GemmArgs {
   int M, N, K, kps, accumulate;
 };
 
+constexpr int kGemmUnroll = 4;
+
 __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int m0 = (blockIdx.y * 2 + (wave >> 1)) * 16;
@@ -101,16 +103,22 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
   const int am = m0 + r, bn = n0 + r;
   const bool arow = am < g.M, bcol = bn < g.N;
   v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int k = kbeg; k < kend; k += 16) {
-    v4s a, b;
+  // kGemmUnroll K-steps per round: all their operand loads are issued before the
+  // first MFMA of the round, so a round costs one memory latency, not kGemmUnroll.
+  for (int k0 = kbeg; k0 < kend; k0 += 16 * kGemmUnroll) {
+    v4s a[kGemmUnroll], b[kGemmUnroll];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int kk = k + kq + j;
-      const bool kin = kk < kend;
-      a[j] = (arow && kin) ? (short)g.A[(long)am * g.sam + (long)kk * g.sak] : (short)0;
-      b[j] = (bcol && kin) ? (short)g.B[(long)kk * g.sbk + (long)bn * g.sbn] : (short)0;
+    for (int u = 0; u < kGemmUnroll; ++u) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int kk = k0 + 16 * u + kq + j;
+        const bool kin = kk < kend;
+        a[u][j] = (arow && kin) ? (short)g.A[(long)am * g.sam + (long)kk * g.sak] : (short)0;
+        b[u][j] = (bcol && kin) ? (short)g.B[(long)kk * g.sbk + (long)bn * g.sbn] : (short)0;
+      }
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < kGemmUnroll; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u], acc, 0, 0, 0);
   }
   const int cn = n0 + (lane & 15);
   if (cn >= g.N) return;
@@ -379,11 +387,16 @@ kf_status gemm(hipStream_t s, const uint16_t* A, long sam, long sak, const uint1
 const ConvShape& conv(int l) { return kConv[l]; }
 int kcp(int l) { return l == 0 ? 32 : kConv[l].kc; }  // im2col row length (layer 1 padded to 32)
 
-kf_status forward(kf_prior* P, const float* frames, const int32_t* pairs, const float* prev, int B, hipStream_t s) {
-  const ParamLayout& L = layout();
+// BF16 copy of the weights; every call that changes the FP32 parameters refreshes
+// it, so inference (forward) never converts.
+kf_status sync_weights(kf_prior* P, hipStream_t s) {
   hipLaunchKernelGGL(to_bf16, dim3(blocks_for(P->nparams)), dim3(256), 0, s, (const float*)P->params, P->wbf,
                      P->nparams);
-  KF_TRY(check_launch("to_bf16"));
+  return check_launch("to_bf16(weights)");
+}
+
+kf_status forward(kf_prior* P, const float* frames, const int32_t* pairs, const float* prev, int B, hipStream_t s) {
+  const ParamLayout& L = layout();
   for (int l = 0; l < kConvs; ++l) {
     const ConvShape& c = conv(l);
     const long M = (long)B * c.hout * c.wout;
@@ -548,6 +561,8 @@ extern "C" kf_status kf_prior_set_params(kf_prior* P, const float* h, int64_t co
   KF_CHECK_ARG(P && h, "NULL argument");
   KF_CHECK_ARG(count == P->nparams, "parameter count mismatch");
   KF_HIP(hipMemcpy(P->params, h, count * 4, hipMemcpyHostToDevice));
+  KF_TRY(sync_weights(P, nullptr));
+  KF_HIP(hipDeviceSynchronize());
   return KF_OK;
 }
 
@@ -613,5 +628,6 @@ extern "C" kf_status kf_prior_train_step(kf_prior* P, const float* frames, const
   const float c1 = 1.0f - (float)pow(0.9, (double)P->step), c2 = 1.0f - (float)pow(0.999, (double)P->step);
   hipLaunchKernelGGL(adam, dim3(blocks_for(P->nparams)), dim3(256), 0, S(s), P->params, (const float*)P->grad,
                      P->adam_m, P->adam_v, P->nparams, lr, b1, b2, 1e-8f, c1, c2);
-  return check_launch("adam");
+  KF_TRY(check_launch("adam"));
+  return sync_weights(P, S(s));
 }
