@@ -337,7 +337,19 @@ __global__ __launch_bounds__(kSolveThreads) void icp_solve(IcpArgs a, int nb) {
     return;
   }
   __shared__ double tx[6], tcs[6], tD[16], tT[16], tN[16];
+  __shared__ double W[6][6], Lm[6][6];
   __shared__ int tok;
+  // Cholesky, right-looking and lane-parallel: step j takes the pivot on lane 0,
+  // the column below it on lanes j+1..5 and the trailing update on one lane per
+  // entry.  Every entry receives the same subtractions in the same order as the
+  // serial left-looking loop (cholesky_solve6), so the factor has the same bits.
+  if (tid < 36) {
+    const int i = tid / 6, k = tid - 6 * (tid / 6);
+    const int r = i < k ? i : k, c = i < k ? k : i;
+    W[i][k] = sum[r * (11 - r) / 2 + c];
+  } else if (tid >= 64 && tid < 80) {
+    tT[tid - 64] = a.pose->T_wc[tid - 64];
+  }
   if (tid == 0) {
     const double inl = sum[28], cand = sum[29];
     kf_icp_stats* st = a.stats;
@@ -347,16 +359,49 @@ __global__ __launch_bounds__(kSolveThreads) void icp_solve(IcpArgs a, int nb) {
     st->residual_rms = ni > 0 ? sqrt(sum[27] / inl) : 0.0;
     st->level_residual[a.level] = st->residual_rms;
     st->level_inliers[a.level] = ni;
-    double x[6];
-    const bool ok = ni >= kMinInliers && cholesky_solve6(sum, sum + 21, kPivotRelEps, x);
-    tok = ok ? 1 : 0;
-    for (int q = 0; q < 6; ++q) tx[q] = ok ? x[q] : 0.0;
-    if (!ok) {
-      a.ctrl->stop = 1;
-      st->status = 1;
+    tok = ni >= kMinInliers ? 1 : 0;
+  }
+  __syncthreads();
+  double thr = 0.0;
+  {
+    double dmax = 0.0;
+    for (int i = 0; i < 6; ++i) dmax = W[i][i] > dmax ? W[i][i] : dmax;
+    thr = kPivotRelEps * dmax;
+  }
+  for (int j = 0; j < 6 && tok; ++j) {  // tok is read from LDS after a barrier: uniform
+    if (tid == 0) {
+      const double d = W[j][j];
+      if (!(d > thr)) tok = 0; else Lm[j][j] = sqrt(d);
     }
-  } else if (tid >= 64 && tid < 80) {
-    tT[tid - 64] = a.pose->T_wc[tid - 64];
+    __syncthreads();
+    if (!tok) break;
+    if (tid > j && tid < 6) Lm[tid][j] = W[tid][j] / Lm[j][j];
+    __syncthreads();
+    if (tid < 36) {
+      const int i = tid / 6, k = tid - 6 * (tid / 6);
+      if (k > j && i >= k) W[i][k] = W[i][k] - Lm[i][j] * Lm[k][j];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (tok) {  // substitutions (serial, same order as cholesky_solve6)
+      const double* g = sum + 21;
+      double y[6], x[6];
+      for (int i = 0; i < 6; ++i) {
+        double s = g[i];
+        for (int m = 0; m < i; ++m) s = s - Lm[i][m] * y[m];
+        y[i] = s / Lm[i][i];
+      }
+      for (int i = 5; i >= 0; --i) {
+        double s = y[i];
+        for (int m = i + 1; m < 6; ++m) s = s - Lm[m][i] * x[m];
+        x[i] = s / Lm[i][i];
+      }
+      for (int q = 0; q < 6; ++q) tx[q] = x[q];
+    } else {
+      a.ctrl->stop = 1;
+      a.stats->status = 1;
+    }
   }
   __syncthreads();
   if (tok) {  // uniform
