@@ -11,12 +11,16 @@ Frames 0..W-1 warm up, frames W..W+K-1 are timed.
 Variants on the same stream: plain FP32 volume (8 B/voxel), packed FP16+u16
 volume (4 B/voxel), and FP16 + the learned pose prior (ICP warm-started with
 4/3/3 iterations, plain 10/5/4 fallback) when trained weights are present;
-`value` is the fps of the last of these that ran (learned + FP16 normally).  A stage-timing pass
-(hipEvents on the pipeline's stream) gives per-kernel times, the sweep's
-algorithmic bytes (U*2*B_v + P*4 with U from the sweep's own update counter) and
-its roofline fraction.  The CPU baseline runs the oracle (OpenMP restatement of
-the reference's per-voxel / per-pixel bodies) on a bounded sample of the FP32
-stream on the host cores.
+`value` is the fps of the last of these that ran (learned + FP16 normally).  A
+stage-timing pass (hipEvents on the pipeline's stream around each kernel group)
+gives per-kernel times; with the kernels' algorithmic bytes (sweep U*2*B_v + P*4
+with U from the sweep's own update counter, raycast P*(S*B_t + 24) with S from its
+step counter, ICP sum over iterations of P_l*49 + partial slots) they give achieved
+GB/s and the fraction of the 8 TB/s HBM peak per kernel; `roofline` is the longer
+of the sweep and the raycast.  `traffic` comes from the committed rocprofv3
+FETCH_SIZE / WRITE_SIZE summary of the same workload.  The CPU baseline runs the
+oracle (OpenMP restatement of the reference's per-voxel / per-pixel bodies) on a
+bounded sample of the FP32 stream on the host cores.
 
 Prints ONE JSON line.  `--gpus N` with N > 1 is not supported (one GPU).
 """
@@ -160,6 +164,57 @@ def stage_run(kf, synth, stream, dd, preset: str, W: int, S: int, use_prior: boo
     return {"ms": ms, "U": U, "steps": steps, "desc": desc, "observed_voxels": float(vstats[0])}
 
 
+def kernel_rooflines(kf, d, sr: dict, prior: bool) -> dict:
+    """Algorithmic bytes per launch / average launch time for the sweep, the raycast
+    and ICP (per frame), from the stage-timing run (hipEvents on the pipeline stream)."""
+    P = d.intr.width * d.intr.height
+    n = d.volume.n
+    Bv = 4 if d.volume.format == kf.KF_VOXEL_F16 else 8
+    ms = sr["ms"]
+    out = {}
+    sweep_bytes = float((sr["U"] * 2 * Bv + P * 4).mean())
+    t = float(ms[:, 3].mean())
+    out["integrate"] = {"kernel": f"integrate<{'FP16' if Bv == 4 else 'FP32'}> (TSDF sweep)",
+                        "algorithmic_bytes": sweep_bytes, "avg_launch_ms": t,
+                        "updated_voxels_per_launch": float(sr["U"].mean()),
+                        "upper_bound_bytes": float(n ** 3 * 2 * Bv + P * 4)}
+    S_bar = float(sr["steps"].mean() / P)
+    ray_bytes = P * (S_bar * (Bv // 2) + 24)
+    t = float(ms[:, 4].mean())
+    out["raycast"] = {"kernel": f"raycast<{'FP16' if Bv == 4 else 'FP32'}>", "algorithmic_bytes": ray_bytes,
+                      "avg_launch_ms": t, "mean_steps_per_ray": S_bar}
+    icp = d.icp_prior if prior else d.icp
+    icp_bytes = 0.0
+    for lvl in range(icp.levels):
+        Pl = P >> (2 * lvl)
+        blocks = min(1024, -(-Pl // (256 * (2 if lvl == 0 else 1))))
+        icp_bytes += icp.iterations[lvl] * (Pl * 49 + 2 * blocks * 256)
+    t = float(ms[:, 1].mean())
+    out["icp"] = {"kernel": "guess + icp_partials/icp_solve (per frame, "
+                            f"{'/'.join(str(icp.iterations[i]) for i in range(icp.levels))} iterations)",
+                  "algorithmic_bytes": icp_bytes, "avg_launch_ms": t}
+    for k in out.values():
+        k["achieved_GBs"] = round(k["algorithmic_bytes"] / (k["avg_launch_ms"] * 1e-3) / 1e9, 1)
+        k["frac"] = round(k["achieved_GBs"] / HBM_PEAK_GBS, 4)
+    return out
+
+
+def profiled_traffic(kernel: str, Bv: int) -> dict | None:
+    """HBM bytes per dispatch of the kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (scaled by the calibration kernels)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    key = f"{kernel}<{1 if Bv == 4 else 0}>"
+    k = t.get("kernels", {}).get(key)
+    if not k:
+        return None
+    return {"bytes": k["fetch_bytes"] + k["write_bytes"], "source": f"profiles/traffic.json[{key}]"}
+
+
 def cpu_baseline(orc, synth, stream, preset: str, frames: int) -> dict:
     desc = synth.pipeline_desc(preset)
     p = orc.Pipeline(desc, 0)
@@ -234,13 +289,11 @@ def main() -> int:
     n = d.volume.n
     Bv = 4 if d.volume.format == kf.KF_VOXEL_F16 else 8
     P = d.intr.width * d.intr.height
-    sweep_ms = sr["ms"][:, 3]
-    alg_bytes = sr["U"] * 2 * Bv + P * 4
-    achieved = float(alg_bytes.mean() / (sweep_ms.mean() * 1e-3) / 1e9)
-    upper = float((n ** 3 * 2 * Bv + P * 4) / (sweep_ms.mean() * 1e-3) / 1e9)
-    ray_ms = sr["ms"][:, 4]
-    S_bar = float(sr["steps"].mean() / P)
-    ray_bytes = P * (S_bar * (2 if Bv == 4 else 4) + 24)
+    stage = sr["ms"].mean(0)
+    kernels = kernel_rooflines(kf, d, sr, head_prior)
+    dominant = max(("integrate", "raycast"), key=lambda k: kernels[k]["avg_launch_ms"])
+    dom = kernels[dominant]
+    traffic = profiled_traffic(dominant, Bv)
     pre = synth.PRESETS[head_preset]
 
     out = {
@@ -267,25 +320,23 @@ def main() -> int:
             "variant": head,
         },
         "roofline": {
-            "kernel": "integrate (TSDF sweep)",
+            "kernel": dom["kernel"],
             "bound": "hbm",
-            "achieved": round(achieved, 1),
+            "achieved": dom["achieved_GBs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "algorithmic_bytes_per_launch": float(alg_bytes.mean()),
-            "updated_voxels_per_launch": float(sr["U"].mean()),
-            "avg_launch_ms": float(sweep_ms.mean()),
-            "upper_bound_gbs_if_all_voxels": round(upper, 1),
+            "frac": round(dom["achieved_GBs"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic["bytes"] if traffic else None,
+            "traffic_source": traffic["source"] if traffic else None,
+            "algorithmic_bytes_per_launch": dom["algorithmic_bytes"],
+            "avg_launch_ms": dom["avg_launch_ms"],
         },
+        "kernels": kernels,
         "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in res.items()},
         "observed_voxels_after_stage_run": sr["observed_voxels"],
-        "stages_ms": dict(zip(["preprocess", "icp", "column_spans", "sweep", "raycast"],
-                              [round(float(x), 4) for x in sr["ms"].mean(0)])),
-        "raycast": {"mean_steps_per_ray": round(S_bar, 2), "algorithmic_GBs":
-                    round(ray_bytes / (ray_ms.mean() * 1e-3) / 1e9, 1)},
+        "stages_ms": dict(zip(["preprocess", "guess+icp", "column_spans", "sweep", "raycast", "model_pyramid"],
+                              [round(float(x), 4) for x in stage])),
     }
     if "plain" in res:
         for name in ("fp16", "learned_fp16"):

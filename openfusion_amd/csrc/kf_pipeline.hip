@@ -256,10 +256,10 @@ kf_status record_track_frame(kf_pipeline* p, bool timed) {
   KF_TRY(launch_integrate(p->vol, d.intr, d.integ, p->live.raylen, &p->d_poses[kCur], upd, nullptr, s,
                           timed ? p->ev[3] : nullptr));
   if (timed) KF_HIP(hipEventRecord(p->ev[4], s));
-  KF_TRY(launch_raycast(p->vol, d.intr, d.ray, &p->d_poses[kCur], p->model, steps, s));
+  KF_TRY(launch_raycast(p->vol, d.intr, d.ray, &p->d_poses[kCur], p->model, steps, s, timed ? p->ev[5] : nullptr));
   hipLaunchKernelGGL(pose_shift, dim3(1), dim3(64), 0, s, p->d_poses, p->d_traj, p->d_traj_count);
   KF_TRY(check_launch("pose_shift"));
-  if (timed) KF_HIP(hipEventRecord(p->ev[5], s));
+  if (timed) KF_HIP(hipEventRecord(p->ev[6], s));
   return KF_OK;
 }
 

@@ -302,7 +302,7 @@ __global__ __launch_bounds__(256) void subsample_maps(SubArgs s) {
 
 kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_raycast_params& p,
                          const kf_pose* d_pose, kf_frame_maps& model, unsigned long long* d_steps,
-                         hipStream_t s) {
+                         hipStream_t s, hipEvent_t after_raycast) {
   KF_CHECK_ARG(d_pose != nullptr, "pose is NULL");
   KF_CHECK_ARG(model.levels >= 1 && model.levels <= KF_MAX_LEVELS, "model levels must be 1..3");
   KF_CHECK_ARG(p.step_far > 0.0f && p.step_near > 0.0f && p.max_steps > 0, "bad march steps");
@@ -346,6 +346,7 @@ kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_r
   else
     hipLaunchKernelGGL(raycast<KF_VOXEL_F32>, dim3(grid), dim3(256), 0, s, a);
   KF_TRY(check_launch("raycast"));
+  if (after_raycast) KF_HIP(hipEventRecord(after_raycast, s));
   if (model.levels > 1) {
     SubArgs sa;
     sa.l0 = model.level[0];
