@@ -90,36 +90,86 @@ struct GemmArgs {
   int M, N, K, kps, accumulate;
 };
 
-constexpr int kGemmUnroll = 4;
+// Each 256-thread block computes a 32x32 tile of C (four 16x16 wave tiles on
+// v_mfma_f32_16x16x16bf16_1k).  K is staged through LDS in chunks of kKc: the block
+// loads the 32 x chunk slices of A and B with coalesced accesses (adjacent threads
+// take adjacent addresses along whichever of K or M/N is unit-stride), then runs
+// the chunk's MFMAs from LDS.  The MFMA sequence over k (steps of 16, zeros beyond
+// K) is the same as a direct k loop, so results do not depend on the chunking.
+constexpr int kKc = 512;
+constexpr int kPitch = kKc + 8;  // LDS row pitch (elements): rows start on different banks
+
+// Stages a 32 x kpad operand slice into T[32][kPitch] (row = m or n, column = k)
+// from X(row, k) = X[row * s_row + k * s_k], zero outside rows < nrows, k < kn.
+__device__ __forceinline__ void stage(uint16_t (*T)[kPitch], const uint16_t* X, long s_row, long s_k, int row0,
+                                      int nrows, long k0, int kn, int kpad) {
+  const int tid = threadIdx.x;
+  if (s_k == 1) {  // K contiguous: thread e takes (row, k) = (e / kpad, e % kpad)
+    const int total = 32 * kpad;
+    for (int e0 = tid; e0 < total; e0 += 256 * 4) {
+      uint16_t v[4];
+      int rr[4], kk[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = e0 + 256 * u;
+        rr[u] = e / kpad;
+        kk[u] = e - rr[u] * kpad;
+        const bool in = e < total && row0 + rr[u] < nrows && kk[u] < kn;
+        v[u] = in ? X[(long)(row0 + rr[u]) * s_row + (k0 + kk[u])] : (uint16_t)0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (e0 + 256 * u < total) T[rr[u]][kk[u]] = v[u];
+    }
+  } else {  // rows contiguous (or general): thread e takes (k, row) = (e / 32, e % 32)
+    const int total = 32 * kpad;
+    for (int e0 = tid; e0 < total; e0 += 256 * 4) {
+      uint16_t v[4];
+      int rr[4], kk[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = e0 + 256 * u;
+        kk[u] = e >> 5;
+        rr[u] = e & 31;
+        const bool in = e < total && row0 + rr[u] < nrows && kk[u] < kn;
+        v[u] = in ? X[(long)(row0 + rr[u]) * s_row + (k0 + kk[u]) * s_k] : (uint16_t)0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (e0 + 256 * u < total) T[rr[u]][kk[u]] = v[u];
+    }
+  }
+}
 
 __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
+  __shared__ __align__(16) uint16_t As[32][kPitch];
+  __shared__ __align__(16) uint16_t Bt[32][kPitch];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int m0 = (blockIdx.y * 2 + (wave >> 1)) * 16;
-  const int n0 = (blockIdx.x * 2 + (wave & 1)) * 16;
-  if (m0 >= g.M || n0 >= g.N) return;  // wave-uniform
+  const int mt = blockIdx.y * 32, nt = blockIdx.x * 32;
+  const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
   const int kbeg = blockIdx.z * g.kps;
   const int kend = min(g.K, kbeg + g.kps);
   const int r = lane & 15, kq = (lane >> 4) * 4;
-  const int am = m0 + r, bn = n0 + r;
-  const bool arow = am < g.M, bcol = bn < g.N;
   v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-  // kGemmUnroll K-steps per round: all their operand loads are issued before the
-  // first MFMA of the round, so a round costs one memory latency, not kGemmUnroll.
-  for (int k0 = kbeg; k0 < kend; k0 += 16 * kGemmUnroll) {
-    v4s a[kGemmUnroll], b[kGemmUnroll];
-#pragma unroll
-    for (int u = 0; u < kGemmUnroll; ++u) {
+  for (int kc0 = kbeg; kc0 < kend; kc0 += kKc) {
+    const int kn = min(kKc, kend - kc0);
+    const int kpad = (kn + 15) & ~15;
+    stage(As, g.A, g.sam, g.sak, mt, g.M, kc0, kn, kpad);
+    stage(Bt, g.B, g.sbn, g.sbk, nt, g.N, kc0, kn, kpad);
+    __syncthreads();
+    for (int kk = 0; kk < kpad; kk += 16) {
+      v4s a, b;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int kk = k0 + 16 * u + kq + j;
-        const bool kin = kk < kend;
-        a[u][j] = (arow && kin) ? (short)g.A[(long)am * g.sam + (long)kk * g.sak] : (short)0;
-        b[u][j] = (bcol && kin) ? (short)g.B[(long)kk * g.sbk + (long)bn * g.sbn] : (short)0;
+        a[j] = (short)As[wm + r][kk + kq + j];
+        b[j] = (short)Bt[wn + r][kk + kq + j];
       }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
-#pragma unroll
-    for (int u = 0; u < kGemmUnroll; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u], acc, 0, 0, 0);
+    __syncthreads();
   }
+  const int m0 = mt + wm, n0 = nt + wn;
+  if (m0 >= g.M || n0 >= g.N) return;  // wave-uniform
   const int cn = n0 + (lane & 15);
   if (cn >= g.N) return;
 #pragma unroll

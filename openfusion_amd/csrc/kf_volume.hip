@@ -279,13 +279,25 @@ __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
     unsigned long long occ_bits = 0ull;  // z-bricks receiving f != 1 from this lane
     // One voxel per lane per step: a wave covers 64 consecutive z (256 B of FP16
     // or 512 B of FP32 voxels, fully coalesced), so few lanes idle at span ends.
-    for (int zb = z0; zb <= z1; zb += kWave) {
+    // Group decisions for all 64-voxel groups of the column at once, group i on
+    // lane i (n <= 2048: at most 32 groups), so their tile loads overlap.
+    unsigned long long skip = 0ull, free_space = 0ull;
+    if (a.mip4) {
+      const int ng = (z1 - z0) / kWave + 1;
+      int dec = 0;
+      if (lane < ng) {
+        const int ga = z0 + lane * kWave;
+        dec = group_test(a, ax, ay, az, P.r, ga, min(ga + kWave - 1, z1));
+      }
+      skip = __ballot(dec == 1);
+      free_space = __ballot(dec == 2);
+    }
+    for (int zb = z0, gi = 0; zb <= z1; zb += kWave, ++gi) {
+      if ((skip >> gi) & 1ull) continue;  // no voxel of the group is updated
+      const bool g_free = (free_space >> gi) & 1ull;  // every voxel is updated as free space
       const int z = zb + lane;
-      // wave-uniform decision for the 64 voxels (see group_test)
-      const int g = a.mip4 ? group_test(a, ax, ay, az, P.r, zb, min(zb + kWave - 1, z1)) : 0;
-      if (g == 1) continue;
       float fs = 1.0f;
-      const bool upd = z <= z1 && (g == 2 || voxel_sample(a, ax, ay, az, P.r, z, fs));
+      const bool upd = z <= z1 && (g_free || voxel_sample(a, ax, ay, az, P.r, z, fs));
       if (upd) {
         ++count;
         const size_t idx = col + (size_t)z;
