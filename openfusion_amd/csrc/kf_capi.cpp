@@ -268,20 +268,12 @@ extern "C" kf_status kf_volume_create(const kf_volume_desc* d, kf_volume** out) 
     delete v;
     return hip_fail(e, "hipMalloc(spans)");
   }
-  e = hipMalloc(&v->d_rlmip, (size_t)(kf::kRlMip4Cells + kf::kRlMip7Cells) * sizeof(float2));
-  if (e != hipSuccess) {
-    (void)hipFree(v->d_voxels);
-    (void)hipFree(v->d_spans);
-    delete v;
-    return hip_fail(e, "hipMalloc(raylen tiles)");
-  }
   if (d->n / 8 <= kf::kOccMaxBricks) {
     v->occ_nb = d->n / 8;
     e = hipMalloc(&v->d_occ, (size_t)v->occ_nb * v->occ_nb * sizeof(unsigned long long));
     if (e != hipSuccess) {
       (void)hipFree(v->d_voxels);
       (void)hipFree(v->d_spans);
-      (void)hipFree(v->d_rlmip);
       delete v;
       return hip_fail(e, "hipMalloc(occupancy)");
     }
@@ -295,7 +287,6 @@ extern "C" kf_status kf_volume_create(const kf_volume_desc* d, kf_volume** out) 
     (void)hipFree(v->d_voxels);
     (void)hipFree(v->d_spans);
     (void)hipFree(v->d_occ);
-    (void)hipFree(v->d_rlmip);
     delete v;
     return st;
   }
@@ -308,7 +299,6 @@ extern "C" kf_status kf_volume_destroy(kf_volume* v) {
   (void)hipFree(v->d_voxels);
   (void)hipFree(v->d_spans);
   (void)hipFree(v->d_occ);
-  (void)hipFree(v->d_rlmip);
   delete v;
   return KF_OK;
 }

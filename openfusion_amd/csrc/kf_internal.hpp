@@ -55,9 +55,6 @@ struct kf_volume {
   unsigned long long* d_occ = nullptr;
   int occ_nb = 0;                // bricks per axis (0: no map)
   mutable bool occ_stale = false;  // voxels may have been written outside the library
-  // Raylen (min, max) tiles of the frame being integrated (16 and 128 pixels),
-  // for images up to kRlMipMaxDim on a side.
-  float2* d_rlmip = nullptr;
 };
 
 // ICP scratch and per-iteration control block (device memory).
@@ -93,9 +90,6 @@ kf_status launch_volume_stats(const kf_volume* v, double* d_out, hipStream_t s);
 // Recomputes the occupancy map from the voxels when it is stale (no-op otherwise).
 kf_status ensure_occupancy(const kf_volume* v, hipStream_t s);
 constexpr int kOccMaxBricks = 64;  // bricks per axis held in one 64-bit word
-constexpr int kRlMipMaxDim = 4096;
-constexpr int kRlMip4Cells = (kRlMipMaxDim / 16) * (kRlMipMaxDim / 16);
-constexpr int kRlMip7Cells = (kRlMipMaxDim / 128) * (kRlMipMaxDim / 128);
 kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_raycast_params& p,
                          const kf_pose* d_pose, kf_frame_maps& model, unsigned long long* d_steps,
                          hipStream_t s, hipEvent_t after_raycast = nullptr);

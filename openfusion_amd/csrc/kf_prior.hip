@@ -96,15 +96,37 @@ struct GemmArgs {
 // take adjacent addresses along whichever of K or M/N is unit-stride), then runs
 // the chunk's MFMAs from LDS.  The MFMA sequence over k (steps of 16, zeros beyond
 // K) is the same as a direct k loop, so results do not depend on the chunking.
-constexpr int kKc = 512;
+constexpr int kKc = 576;        // whole K of every convolution layer fits one chunk
 constexpr int kPitch = kKc + 8;  // LDS row pitch (elements): rows start on different banks
+constexpr int kVecPerThread = (32 * kKc / 8 + 255) / 256;  // 16-byte vectors per thread and operand
 
 // Stages a 32 x kpad operand slice into T[32][kPitch] (row = m or n, column = k)
 // from X(row, k) = X[row * s_row + k * s_k], zero outside rows < nrows, k < kn.
 __device__ __forceinline__ void stage(uint16_t (*T)[kPitch], const uint16_t* X, long s_row, long s_k, int row0,
                                       int nrows, long k0, int kn, int kpad) {
   const int tid = threadIdx.x;
-  if (s_k == 1) {  // K contiguous: thread e takes (row, k) = (e / kpad, e % kpad)
+  const bool vec = s_k == 1 && (s_row & 7) == 0 && (k0 & 7) == 0 && (kn & 7) == 0 &&
+                   ((reinterpret_cast<uintptr_t>(X) & 15) == 0);
+  if (vec) {  // 16-byte loads of 8 consecutive k, all issued before any LDS store
+    const int vpr = kpad >> 3;  // vectors per row (the last one may be k padding)
+    const int total = 32 * vpr;
+    uint4 v[kVecPerThread];
+#pragma unroll
+    for (int u = 0; u < kVecPerThread; ++u) {
+      const int e = tid + 256 * u;
+      const int r = e / vpr, c = e - (e / vpr) * vpr;
+      const bool in = e < total && row0 + r < nrows && 8 * c < kn;
+      v[u] = in ? *reinterpret_cast<const uint4*>(X + (long)(row0 + r) * s_row + k0 + 8 * c) : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < kVecPerThread; ++u) {
+      const int e = tid + 256 * u;
+      if (e < total) {
+        const int r = e / vpr, c = e - (e / vpr) * vpr;
+        *reinterpret_cast<uint4*>(&T[r][8 * c]) = v[u];
+      }
+    }
+  } else if (s_k == 1) {  // K contiguous: thread e takes (row, k) = (e / kpad, e % kpad)
     const int total = 32 * kpad;
     for (int e0 = tid; e0 < total; e0 += 256 * 4) {
       uint16_t v[4];

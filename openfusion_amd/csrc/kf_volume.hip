@@ -40,137 +40,12 @@ struct SweepArgs {
   float dmin, dmax;
   float trunc, inv_trunc, maxw;
   float band;  // T + max(0.01 T, 0.1 mm): half-width of the band where the square root is taken
-  const float2* mip4;  // raylen (min, max) over 16x16-pixel tiles, row-major, mip4_w per row
-  const float2* mip7;  // raylen (min, max) over 128x128-pixel tiles, mip7_w per row
-  int mip4_w, mip7_w;
 };
 
 __global__ __launch_bounds__(256) void volume_fill(uint4* p, size_t n16, uint4 word) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n16; i += stride) p[i] = word;
-}
-
-// Raylen (min, max) tiles for the sweep's group tests: one block per 128x128-pixel
-// tile; thread t covers 16 x 4 pixels of the 16x16 tile t / 4 (float4 loads), four
-// threads combine into that tile's entry, and the block combines the 64 tiles into
-// the 128x128 entry.  Invalid pixels (raylen 0) count: the minimum is then 0.
-__global__ __launch_bounds__(256) void raylen_tiles(const float* rl, int W, int H, float2* mip4, int mip4_w,
-                                                    float2* mip7, int mip7_w) {
-  __shared__ float smin[64], smax[64];
-  const int t = threadIdx.x, cell = t >> 2, q = t & 3;
-  const int x0 = blockIdx.x * 128 + (cell & 7) * 16, y0 = blockIdx.y * 128 + (cell >> 3) * 16 + q * 4;
-  float mn = 3.0e38f, mx = 0.0f;
-  bool any = false;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int y = y0 + r;
-    if (y >= H) continue;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int x = x0 + 4 * c;
-      if (x + 3 < W && (W & 3) == 0) {
-        const float4 v = *reinterpret_cast<const float4*>(rl + (size_t)y * W + x);
-        mn = fminf(fminf(mn, v.x), fminf(v.y, fminf(v.z, v.w)));
-        mx = fmaxf(fmaxf(mx, v.x), fmaxf(v.y, fmaxf(v.z, v.w)));
-        any = true;
-      } else {
-        for (int k = 0; k < 4; ++k)
-          if (x + k < W) {
-            const float v = rl[(size_t)y * W + x + k];
-            mn = fminf(mn, v);
-            mx = fmaxf(mx, v);
-            any = true;
-          }
-      }
-    }
-  }
-  if (!any) mn = 3.0e38f;
-  // combine the 4 threads of a 16x16 tile (lanes 4c..4c+3 of the same wave)
-  mn = fminf(mn, __shfl_xor(mn, 1, kWave));
-  mx = fmaxf(mx, __shfl_xor(mx, 1, kWave));
-  mn = fminf(mn, __shfl_xor(mn, 2, kWave));
-  mx = fmaxf(mx, __shfl_xor(mx, 2, kWave));
-  const int tx4 = x0 >> 4, ty4 = (blockIdx.y * 128 + (cell >> 3) * 16) >> 4;
-  if (q == 0) {
-    smin[cell] = mn;
-    smax[cell] = mx;
-    if (tx4 < mip4_w && ty4 * 16 < H) mip4[ty4 * mip4_w + tx4] = make_float2(mn, mx);
-  }
-  __syncthreads();
-  if (t < 64) {
-    float a = smin[t], b = smax[t];
-    for (int o = 32; o > 0; o >>= 1) {
-      a = fminf(a, __shfl_xor(a, o, kWave));
-      b = fmaxf(b, __shfl_xor(b, o, kWave));
-    }
-    if (t == 0) mip7[blockIdx.y * mip7_w + blockIdx.x] = make_float2(a, b);
-  }
-}
-
-// Raylen (min, max) over the pixel box [u0, u1] x [v0, v1] (inside the image) from
-// the tile pyramid, or false if the box is wider than 129 pixels.  A box of at
-// most 17 (129) pixels spans at most 2x2 tiles of 16 (128) pixels.
-__device__ __forceinline__ bool raylen_box(const SweepArgs& a, int u0, int u1, int v0, int v1, float& mn,
-                                           float& mx) {
-  const int ext = max(u1 - u0, v1 - v0) + 1;
-  const float2* m;
-  int sh, w;
-  if (ext <= 17) {
-    m = a.mip4;
-    sh = 4;
-    w = a.mip4_w;
-  } else if (ext <= 129) {
-    m = a.mip7;
-    sh = 7;
-    w = a.mip7_w;
-  } else {
-    return false;
-  }
-  const int cx0 = u0 >> sh, cx1 = u1 >> sh, cy0 = v0 >> sh, cy1 = v1 >> sh;
-  const float2 p00 = m[cy0 * w + cx0], p01 = m[cy0 * w + cx1], p10 = m[cy1 * w + cx0], p11 = m[cy1 * w + cx1];
-  mn = fminf(fminf(p00.x, p01.x), fminf(p10.x, p11.x));
-  mx = fmaxf(fmaxf(p00.y, p01.y), fmaxf(p10.y, p11.y));
-  return true;
-}
-
-// Group test for voxels z in [za, zb] of one column (camera-frame line p(z) =
-// (ax, ay, az) + r(:,2) wz(z)): 0 = decide per voxel, 1 = no voxel is updated (all
-// beyond the surface by more than the band, or outside the image), 2 = every voxel
-// is updated as free space (f = 1).  Margins: 1 px around the projected segment,
-// 1e-4 m in depth, 1 mm in distance — each far above the FP32 error of the
-// per-voxel expressions, so the per-voxel test would reach the same outcome.
-__device__ __forceinline__ int group_test(const SweepArgs& a, float ax, float ay, float az, const float* r, int za,
-                                          int zb) {
-  const float wa = a.oz + ((float)za + 0.5f) * a.h, wb = a.oz + ((float)zb + 0.5f) * a.h;
-  const f3 pa = f3{ax + r[2] * wa, ay + r[5] * wa, az + r[8] * wa};
-  const f3 pb = f3{ax + r[2] * wb, ay + r[5] * wb, az + r[8] * wb};
-  if (!(pa.z > 0.05f && pb.z > 0.05f)) return 0;
-  const float ua = (a.fx * pa.x + a.cx * pa.z) / pa.z, va = (a.fy * pa.y + a.cy * pa.z) / pa.z;
-  const float ub = (a.fx * pb.x + a.cx * pb.z) / pb.z, vb = (a.fy * pb.y + a.cy * pb.z) / pb.z;
-  const float fu0 = floorf(fminf(ua, ub)) - 1.0f, fu1 = ceilf(fmaxf(ua, ub)) + 1.0f;
-  const float fv0 = floorf(fminf(va, vb)) - 1.0f, fv1 = ceilf(fmaxf(va, vb)) + 1.0f;
-  // pixels a voxel of the group can be projected to and still be updated
-  const float lo_u = 2.0f, hi_u = (float)(a.W - 3), lo_v = 2.0f, hi_v = (float)(a.H - 3);
-  if (fu1 < lo_u || fu0 > hi_u || fv1 < lo_v || fv0 > hi_v) return 1;
-  const bool inside = fu0 >= lo_u && fu1 <= hi_u && fv0 >= lo_v && fv1 <= hi_v;
-  const int u0 = (int)fmaxf(fu0, lo_u), u1 = (int)fminf(fu1, hi_u);
-  const int v0 = (int)fmaxf(fv0, lo_v), v1 = (int)fminf(fv1, hi_v);
-  float rmin, rmax;
-  if (!raylen_box(a, u0, u1, v0, v1, rmin, rmax)) return 0;
-  const float da = sqrtf(dot(pa, pa)), db = sqrtf(dot(pb, pb));
-  // closest point of the segment to the camera
-  const f3 d = sub(pb, pa);
-  const float dd = dot(d, d);
-  const float tc = dd > 0.0f ? fminf(fmaxf(-dot(pa, d) / dd, 0.0f), 1.0f) : 0.0f;
-  const f3 pc = f3{pa.x + tc * d.x, pa.y + tc * d.y, pa.z + tc * d.z};
-  const float dmin_seg = fminf(sqrtf(dot(pc, pc)), fminf(da, db));
-  if (dmin_seg > rmax + a.band + 1e-3f) return 1;
-  const float zlo = fminf(pa.z, pb.z), zhi = fmaxf(pa.z, pb.z);
-  if (inside && zlo >= a.dmin + 1e-4f && zhi <= a.dmax - 1e-4f && rmin > 0.0f &&
-      fmaxf(da, db) < rmin - a.band - 1e-3f)
-    return 2;
-  return 0;
 }
 
 // Conservative frustum span per column (FP64).  Margins: 0.05 px in the image
@@ -279,25 +154,10 @@ __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
     unsigned long long occ_bits = 0ull;  // z-bricks receiving f != 1 from this lane
     // One voxel per lane per step: a wave covers 64 consecutive z (256 B of FP16
     // or 512 B of FP32 voxels, fully coalesced), so few lanes idle at span ends.
-    // Group decisions for all 64-voxel groups of the column at once, group i on
-    // lane i (n <= 2048: at most 32 groups), so their tile loads overlap.
-    unsigned long long skip = 0ull, free_space = 0ull;
-    if (a.mip4) {
-      const int ng = (z1 - z0) / kWave + 1;
-      int dec = 0;
-      if (lane < ng) {
-        const int ga = z0 + lane * kWave;
-        dec = group_test(a, ax, ay, az, P.r, ga, min(ga + kWave - 1, z1));
-      }
-      skip = __ballot(dec == 1);
-      free_space = __ballot(dec == 2);
-    }
-    for (int zb = z0, gi = 0; zb <= z1; zb += kWave, ++gi) {
-      if ((skip >> gi) & 1ull) continue;  // no voxel of the group is updated
-      const bool g_free = (free_space >> gi) & 1ull;  // every voxel is updated as free space
+    for (int zb = z0; zb <= z1; zb += kWave) {
       const int z = zb + lane;
-      float fs = 1.0f;
-      const bool upd = z <= z1 && (g_free || voxel_sample(a, ax, ay, az, P.r, z, fs));
+      float fs = 0.0f;
+      const bool upd = z <= z1 && voxel_sample(a, ax, ay, az, P.r, z, fs);
       if (upd) {
         ++count;
         const size_t idx = col + (size_t)z;
@@ -515,11 +375,6 @@ kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integ
   a.mask = d_mask;
   a.occ = v->d_occ;
   a.occ_nb = v->occ_nb;
-  const bool tiles = v->d_rlmip && k0.width <= kRlMipMaxDim && k0.height <= kRlMipMaxDim;
-  a.mip4_w = div_up(k0.width, 16);
-  a.mip7_w = div_up(k0.width, 128);
-  a.mip4 = tiles ? v->d_rlmip : nullptr;
-  a.mip7 = tiles ? v->d_rlmip + kRlMip4Cells : nullptr;
   a.n = d.n;
   a.h = d.extent_m / (float)d.n;
   a.ox = d.origin_m[0];
@@ -540,11 +395,6 @@ kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integ
   const int cols = d.n * d.n;
   hipLaunchKernelGGL(column_spans, dim3(div_up(cols, 256)), dim3(256), 0, s, a);
   KF_TRY(check_launch("column_spans"));
-  if (a.mip4) {
-    hipLaunchKernelGGL(raylen_tiles, dim3(div_up(k0.width, 128), div_up(k0.height, 128)), dim3(256), 0, s, d_raylen,
-                       k0.width, k0.height, v->d_rlmip, a.mip4_w, v->d_rlmip + kRlMip4Cells, a.mip7_w);
-    KF_TRY(check_launch("raylen_tiles"));
-  }
   if (between) KF_HIP(hipEventRecord(between, s));
   const int blocks = std::min(div_up(cols, kWavesPerBlock), kSweepMaxBlocks);
   if (d.format == KF_VOXEL_F16)
