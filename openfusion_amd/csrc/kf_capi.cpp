@@ -270,7 +270,12 @@ extern "C" kf_status kf_volume_create(const kf_volume_desc* d, kf_volume** out) 
   }
   if (d->n / 8 <= kf::kOccMaxBricks) {
     v->occ_nb = d->n / 8;
-    e = hipMalloc(&v->d_occ, (size_t)v->occ_nb * v->occ_nb * sizeof(unsigned long long));
+    v->occ_nsb = (v->occ_nb + 3) / 4;
+    // brick words, then the super-brick words, rounded up to whole 8-byte words (the
+    // clearing kernel writes 8-byte words over both)
+    e = hipMalloc(&v->d_occ, (size_t)v->occ_nb * v->occ_nb * sizeof(unsigned long long) +
+                                 (((size_t)v->occ_nsb * v->occ_nsb + 1) / 2) * sizeof(unsigned long long));
+    if (e == hipSuccess) v->d_occ2 = reinterpret_cast<unsigned*>(v->d_occ + (size_t)v->occ_nb * v->occ_nb);
     if (e != hipSuccess) {
       (void)hipFree(v->d_voxels);
       (void)hipFree(v->d_spans);

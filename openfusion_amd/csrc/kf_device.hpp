@@ -135,8 +135,26 @@ __device__ __forceinline__ unsigned long long occ_zbits(int z) {
 // wave, uniform control flow).  Up to four brick columns receive the bits (the
 // column's own and, on a brick boundary, the one below in x and/or y); a word is
 // only written when bits are missing.
-__device__ __forceinline__ void occ_mark_column(unsigned long long* occ, int nb, int ix, int iy,
-                                                unsigned long long bits, int lane) {
+// Brick occupancy map plus its coarse level: super-bricks of 4x4x4 bricks (32^3
+// voxels), one 32-bit word per (sx, sy) with bit sz, set iff one of its bricks is.
+struct OccMap {
+  unsigned long long* words;  // nb * nb, bit bz
+  unsigned* super;            // nsb * nsb, bit sz
+  int nb, nsb;
+};
+
+// Bit i of the result = OR of bits 4i..4i+3 of m.
+__device__ __forceinline__ unsigned occ_compress4(unsigned long long m) {
+  m |= m >> 1;
+  m |= m >> 2;
+  unsigned r = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) r |= (unsigned)((m >> (4 * i)) & 1ull) << i;
+  return r;
+}
+
+__device__ __forceinline__ void occ_mark_column(const OccMap& occ, int ix, int iy, unsigned long long bits,
+                                                int lane) {
   if (__ballot(bits != 0ull) == 0ull) return;
   const unsigned lo = wave_or_u32((unsigned)bits);
   const unsigned hi = wave_or_u32((unsigned)(bits >> 32));
@@ -146,10 +164,15 @@ __device__ __forceinline__ void occ_mark_column(unsigned long long* occ, int nb,
     const int bx = (lane & 1) ? (((ix & 7) == 0 && bx0 > 0) ? bx0 - 1 : -1) : bx0;
     const int by = (lane & 2) ? (((iy & 7) == 0 && by0 > 0) ? by0 - 1 : -1) : by0;
     if (bx >= 0 && by >= 0) {
-      unsigned long long* w = occ + (bx * nb + by);
+      unsigned long long* w = occ.words + (bx * occ.nb + by);
       const unsigned long long cur = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if ((cur & zm) != zm)
+      if ((cur & zm) != zm) {
         __hip_atomic_fetch_or(w, zm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned sm = occ_compress4(zm);
+        unsigned* sw = occ.super + ((bx >> 2) * occ.nsb + (by >> 2));
+        const unsigned scur = __hip_atomic_load(sw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((scur & sm) != sm) __hip_atomic_fetch_or(sw, sm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
   }
 }

@@ -53,7 +53,9 @@ struct kf_volume {
   // of a trilinear sample whose base voxel lies in the brick).  The sweep sets
   // bits as it writes f != 1; bits are never cleared except by a reset or a rebuild.
   unsigned long long* d_occ = nullptr;
+  unsigned* d_occ2 = nullptr;    // super-brick level (32^3 voxels): occ_nsb^2 words, bit sz
   int occ_nb = 0;                // bricks per axis (0: no map)
+  int occ_nsb = 0;               // super-bricks per axis
   mutable bool occ_stale = false;  // voxels may have been written outside the library
 };
 

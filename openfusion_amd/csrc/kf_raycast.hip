@@ -26,7 +26,8 @@ struct RayArgs {
   kf_level_maps out;
   unsigned long long* steps;
   const unsigned long long* occ;  // brick occupancy map or NULL
-  int occ_nb;
+  const unsigned* occ2;           // its super-brick level (32^3 voxels)
+  int occ_nb, occ_nsb;
   int n;
   float h, inv_h, ox, oy, oz;
   float fx, fy, cx, cy;
@@ -143,7 +144,11 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
   const int per = (nb + kXcds - 1) / kXcds;
   const int xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
   const int logical = xcd * per + slot;
-  if (logical >= nb) return;
+  if (logical >= nb) return;  // whole block
+  __shared__ unsigned s_super[256];  // super-brick words (occ_nsb <= 16)
+  const int nsw = a.occ2 ? a.occ_nsb * a.occ_nsb : 0;
+  if ((int)threadIdx.x < nsw) s_super[threadIdx.x] = a.occ2[threadIdx.x];
+  __syncthreads();
   const int by = logical / a.blocks_x, bx = logical - by * a.blocks_x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int x = bx * 16 + (wave & 1) * 8 + (lane & 7);
@@ -215,7 +220,47 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
     float t = tnear, fp;
     bool ok = sample<FMT>(a, f3{o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z}, fp) && fp >= 0.0f;
     int s = 0;
+    // Free-space skip.  From a far-mode position t (sample 1) whose base voxel lies
+    // in a clear super-brick, advance k far steps at once: the k positions are the
+    // sequential ones (t += step, k additions), and the grid coordinate of a
+    // position is a monotone function of t on each axis, so if the last one is
+    // still in the super-brick (checked with the exact sample() arithmetic) every
+    // intermediate one is too, and each of their samples is exactly 1.  The step
+    // count, the position and the value are those of the step-by-step march.
+    auto skip_free = [&](float& t_, int& s_) {
+      const float gx = ((o.x + t_ * dir.x) - a.ox) * a.inv_h - 0.5f;
+      const float gy = ((o.y + t_ * dir.y) - a.oy) * a.inv_h - 0.5f;
+      const float gz = ((o.z + t_ * dir.z) - a.oz) * a.inv_h - 0.5f;
+      const int sx = (int)gx >> 5, sy = (int)gy >> 5, sz = (int)gz >> 5;
+      if ((s_super[sx * a.occ_nsb + sy] >> sz) & 1u) return;
+      const float lim = (float)(a.n - 1);
+      const float lx = (float)(32 * sx), hx = fminf((float)(32 * sx + 32), lim);
+      const float ly = (float)(32 * sy), hy = fminf((float)(32 * sy + 32), lim);
+      const float lz = (float)(32 * sz), hz = fminf((float)(32 * sz + 32), lim);
+      // approximate exit parameter of the box (grid value G at world (G + 0.5) h + origin)
+      float te = tfar;
+      if (dir.x > 0.0f) te = fminf(te, ((hx + 0.5f) * a.h + a.ox - o.x) / dir.x);
+      if (dir.x < 0.0f) te = fminf(te, ((lx + 0.5f) * a.h + a.ox - o.x) / dir.x);
+      if (dir.y > 0.0f) te = fminf(te, ((hy + 0.5f) * a.h + a.oy - o.y) / dir.y);
+      if (dir.y < 0.0f) te = fminf(te, ((ly + 0.5f) * a.h + a.oy - o.y) / dir.y);
+      if (dir.z > 0.0f) te = fminf(te, ((hz + 0.5f) * a.h + a.oz - o.z) / dir.z);
+      if (dir.z < 0.0f) te = fminf(te, ((lz + 0.5f) * a.h + a.oz - o.z) / dir.z);
+      int k = (int)((te - t_) / a.step_far_m) - 1;
+      k = min(k, a.max_steps - s_);
+      if (k <= 0) return;
+      float tt = t_;
+      for (int j = 0; j < k; ++j) tt = tt + a.step_far_m;
+      const float ex = ((o.x + tt * dir.x) - a.ox) * a.inv_h - 0.5f;
+      const float ey = ((o.y + tt * dir.y) - a.oy) * a.inv_h - 0.5f;
+      const float ez = ((o.z + tt * dir.z) - a.oz) * a.inv_h - 0.5f;
+      if (tt <= tfar && ex >= lx && ex < hx && ey >= ly && ey < hy && ez >= lz && ez < hz) {
+        t_ = tt;
+        s_ += k;
+        nsteps += k;
+      }
+    };
     while (ok && s < a.max_steps) {
+      if (nsw && fp >= 1.0f) skip_free(t, s);
       const bool far = fp >= 1.0f;
       const float st = far ? a.step_far_m : a.step_near_m;
       float tk[kSpec], fk[kSpec];
@@ -320,7 +365,9 @@ kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_r
   a.steps = d_steps;
   KF_TRY(ensure_occupancy(v, s));
   a.occ = v->d_occ;
+  a.occ2 = v->d_occ2;
   a.occ_nb = v->occ_nb;
+  a.occ_nsb = v->occ_nsb;
   a.n = d.n;
   a.h = d.extent_m / (float)d.n;
   a.inv_h = (float)d.n / d.extent_m;

@@ -31,8 +31,7 @@ struct SweepArgs {
   const kf_pose* pose;
   unsigned long long* updated;
   uint32_t* mask;
-  unsigned long long* occ;  // brick occupancy map or NULL
-  int occ_nb;
+  OccMap occ;  // brick occupancy map (occ.words NULL: none)
   int n;
   float h, ox, oy, oz;
   float fx, fy, cx, cy;
@@ -171,7 +170,7 @@ __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
           const float wn = fminf(wo + 1.0f, a.maxw);
           const uint32_t nw = float_to_half_bits(fn) | ((uint32_t)wn << 16);
           if (nw != w) *p = nw;  // saturated free space (f = 1, w = max) is left as it is
-          if (a.occ && (nw & 0xffffu) != 0x3C00u) occ_bits |= occ_zbits(z);
+          if (a.occ.words && (nw & 0xffffu) != 0x3C00u) occ_bits |= occ_zbits(z);
         } else {
           float2* p = static_cast<float2*>(a.vox) + idx;
           const float2 v = *p;
@@ -179,7 +178,7 @@ __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
           const float wn = fminf(v.y + 1.0f, a.maxw);
           if (__float_as_uint(fn) != __float_as_uint(v.x) || __float_as_uint(wn) != __float_as_uint(v.y))
             *p = make_float2(fn, wn);
-          if (a.occ && fn != 1.0f) occ_bits |= occ_zbits(z);
+          if (a.occ.words && fn != 1.0f) occ_bits |= occ_zbits(z);
         }
       }
       if (a.mask) {  // 64 consecutive voxels -> up to three 32-bit mask words
@@ -201,7 +200,7 @@ __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
         }
       }
     }
-    if (a.occ) occ_mark_column(a.occ, a.occ_nb, ix, iy, occ_bits, lane);
+    if (a.occ.words) occ_mark_column(a.occ, ix, iy, occ_bits, lane);
   }
   if (a.updated) {  // one device atomic per block (uniform branch)
     __shared__ int red[kWavesPerBlock];
@@ -268,7 +267,7 @@ __global__ __launch_bounds__(256) void volume_stats(const uint4* p, size_t n16, 
 // columns) reads the whole column with 16-byte loads and marks the z-bricks that
 // hold f != 1.
 template <int FMT>
-__global__ __launch_bounds__(256) void occupancy_rebuild(const void* vox, int n, unsigned long long* occ, int nb) {
+__global__ __launch_bounds__(256) void occupancy_rebuild(const void* vox, int n, OccMap occ) {
   constexpr int V = (FMT == KF_VOXEL_F16) ? 4 : 2;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
   const int ncol = n * n;
@@ -288,7 +287,7 @@ __global__ __launch_bounds__(256) void occupancy_rebuild(const void* vox, int n,
         if (__uint_as_float(q.z) != 1.0f) bits |= occ_zbits(zb + 1);
       }
     }
-    occ_mark_column(occ, nb, ix, iy, bits, lane);
+    occ_mark_column(occ, ix, iy, bits, lane);
   }
 }
 
@@ -318,6 +317,17 @@ kf_status launch_volume_stats(const kf_volume* v, double* d_out, hipStream_t s) 
   return check_launch("volume_stats");
 }
 
+OccMap occ_map(const kf_volume* v) {
+  return OccMap{v->d_occ, v->d_occ2, v->occ_nb, v->occ_nsb};
+}
+
+// Both levels to zero (a fill kernel, so that graph capture stays valid).
+kf_status clear_occupancy(const kf_volume* v, hipStream_t s) {
+  const int words = v->occ_nb * v->occ_nb + (v->occ_nsb * v->occ_nsb + 1) / 2;  // the whole allocation
+  hipLaunchKernelGGL(fill_u64, dim3(div_up(words, 256)), dim3(256), 0, s, v->d_occ, words, 0ull);
+  return check_launch("fill_u64(occupancy)");
+}
+
 kf_status launch_volume_reset(kf_volume* v, hipStream_t s) {
   uint4 word;
   if (v->desc.format == KF_VOXEL_F16) {
@@ -332,28 +342,22 @@ kf_status launch_volume_reset(kf_volume* v, hipStream_t s) {
   hipLaunchKernelGGL(volume_fill, dim3(blocks), dim3(256), 0, s, static_cast<uint4*>(v->d_voxels), n16,
                      word);
   KF_TRY(check_launch("volume_fill"));
-  if (v->d_occ) {  // every voxel is free space: all bricks clear
-    const int words = v->occ_nb * v->occ_nb;
-    hipLaunchKernelGGL(fill_u64, dim3(div_up(words, 256)), dim3(256), 0, s, v->d_occ, words, 0ull);
-    KF_TRY(check_launch("fill_u64"));
-  }
+  if (v->d_occ) KF_TRY(clear_occupancy(v, s));  // every voxel is free space: all bricks clear
   v->occ_stale = false;
   return KF_OK;
 }
 
 kf_status ensure_occupancy(const kf_volume* v, hipStream_t s) {
   if (!v->d_occ || !v->occ_stale) return KF_OK;
-  const int words = v->occ_nb * v->occ_nb;
-  hipLaunchKernelGGL(fill_u64, dim3(div_up(words, 256)), dim3(256), 0, s, v->d_occ, words, 0ull);
-  KF_TRY(check_launch("fill_u64"));
+  KF_TRY(clear_occupancy(v, s));
   const int n = v->desc.n;
   const int blocks = std::min(div_up(n * n, kWavesPerBlock), kSweepMaxBlocks);
   if (v->desc.format == KF_VOXEL_F16)
     hipLaunchKernelGGL(occupancy_rebuild<KF_VOXEL_F16>, dim3(blocks), dim3(256), 0, s,
-                       (const void*)v->d_voxels, n, v->d_occ, v->occ_nb);
+                       (const void*)v->d_voxels, n, occ_map(v));
   else
     hipLaunchKernelGGL(occupancy_rebuild<KF_VOXEL_F32>, dim3(blocks), dim3(256), 0, s,
-                       (const void*)v->d_voxels, n, v->d_occ, v->occ_nb);
+                       (const void*)v->d_voxels, n, occ_map(v));
   KF_TRY(check_launch("occupancy_rebuild"));
   v->occ_stale = false;
   return KF_OK;
@@ -373,8 +377,7 @@ kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integ
   a.pose = d_pose;
   a.updated = d_updated;
   a.mask = d_mask;
-  a.occ = v->d_occ;
-  a.occ_nb = v->occ_nb;
+  a.occ = occ_map(v);
   a.n = d.n;
   a.h = d.extent_m / (float)d.n;
   a.ox = d.origin_m[0];
