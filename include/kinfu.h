@@ -160,6 +160,10 @@ kf_status kf_volume_mark_modified(kf_volume* vol);
  * every step).  Rebuilds a stale map on stream s first. */
 kf_status kf_volume_occupancy(const kf_volume* vol, const uint64_t** d_words, int32_t* bricks_per_axis,
                               kf_stream s);
+/* Its coarse level: super-bricks of 4x4x4 bricks (32^3 voxels), word sx*nsb + sy,
+ * bit sz set iff one of the super-brick's bricks is set (nsb = ceil(n / 32)). */
+kf_status kf_volume_occupancy_coarse(const kf_volume* vol, const uint32_t** d_words, int32_t* per_axis,
+                                     kf_stream s);
 /* Occupancy statistics over every voxel (reads the whole volume once):
  * d_out (device, 4 doubles) = {observed voxels (w > 0), sum of w, sum of f over
  * observed voxels, voxels with f < 0 and w > 0}. */

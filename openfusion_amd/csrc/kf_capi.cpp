@@ -340,6 +340,15 @@ extern "C" kf_status kf_volume_mark_modified(kf_volume* v) {
   return KF_OK;
 }
 
+extern "C" kf_status kf_volume_occupancy_coarse(const kf_volume* v, const uint32_t** d_words, int32_t* per_axis,
+                                                kf_stream s) {
+  KF_CHECK_ARG(v && d_words && per_axis, "NULL argument");
+  KF_TRY(kf::ensure_occupancy(v, S(s)));
+  *d_words = reinterpret_cast<const uint32_t*>(v->d_occ2);
+  *per_axis = v->occ_nsb;
+  return KF_OK;
+}
+
 extern "C" kf_status kf_volume_occupancy(const kf_volume* v, const uint64_t** d_words, int32_t* bricks_per_axis,
                                          kf_stream s) {
   KF_CHECK_ARG(v && d_words && bricks_per_axis, "NULL argument");

@@ -145,6 +145,7 @@ _PROTOS = {
     "kf_volume_mark_modified": (C.c_int, [_VP]),
     "kf_volume_data_const": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(C.c_size_t)]),
     "kf_volume_occupancy": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(C.c_int32), _VP]),
+    "kf_volume_occupancy_coarse": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(C.c_int32), _VP]),
     "kf_volume_export_tsdf": (C.c_int, [_VP, _VP, _VP]),
     "kf_volume_extract_mesh": (C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP]),
     "kf_depth_median": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, C.c_uint16, _VP]),
@@ -464,6 +465,19 @@ class Volume:
             return np.zeros((0, 0, 0), bool)
         words = download_ptr(p.value, np.uint64, nb.value * nb.value).reshape(nb.value, nb.value)
         bits = (words[:, :, None] >> np.arange(nb.value, dtype=np.uint64)[None, None, :]) & np.uint64(1)
+        return bits.astype(bool)
+
+    def occupancy_coarse(self) -> np.ndarray:
+        """Super-brick level (32^3 voxels) as (nsb, nsb, nsb) bool [sx, sy, sz]."""
+        p = _VP()
+        nsb = C.c_int32()
+        check(lib().kf_volume_occupancy_coarse(self.handle, C.byref(p), C.byref(nsb), None),
+              "kf_volume_occupancy_coarse")
+        synchronize()
+        if nsb.value == 0:
+            return np.zeros((0, 0, 0), bool)
+        words = download_ptr(p.value, np.uint32, nsb.value * nsb.value).reshape(nsb.value, nsb.value)
+        bits = (words[:, :, None] >> np.arange(nsb.value, dtype=np.uint32)[None, None, :]) & np.uint32(1)
         return bits.astype(bool)
 
     def stats(self) -> np.ndarray:

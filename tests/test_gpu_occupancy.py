@@ -26,12 +26,28 @@ def exact_occupancy(f: np.ndarray) -> np.ndarray:
     return a
 
 
+def coarse_of(occ: np.ndarray) -> np.ndarray:
+    """Super-brick level: OR over groups of 4x4x4 bricks."""
+    nb = occ.shape[0]
+    nsb = (nb + 3) // 4
+    pad = np.zeros((4 * nsb,) * 3, bool)
+    pad[:nb, :nb, :nb] = occ
+    return pad.reshape(nsb, 4, nsb, 4, nsb, 4).any(axis=(1, 3, 5))
+
+
 def _tsdf(kf, vol) -> np.ndarray:
     raw = vol.download_raw()
     n = vol.desc.n
     if vol.desc.format == kf.KF_VOXEL_F16:
         return (raw.view(np.uint32) & 0xFFFF).astype(np.uint16).view(np.float16).astype(np.float32).reshape(n, n, n)
     return raw.view(np.float32)[0::2].reshape(n, n, n)
+
+
+def test_coarse_reference():
+    occ = np.zeros((8, 8, 8), bool)
+    occ[5, 0, 3] = True
+    c = coarse_of(occ)
+    assert c.shape == (2, 2, 2) and sorted(map(tuple, np.argwhere(c))) == [(1, 0, 0)]
 
 
 def test_exact_occupancy_reference():
@@ -61,6 +77,7 @@ def test_rebuild_is_exact(gpu, kf, fmt):
     assert occ.shape == (8, 8, 8)
     assert 0 < ref.sum() < ref.size
     assert np.array_equal(occ, ref)
+    assert np.array_equal(vol.occupancy_coarse(), coarse_of(ref))
 
 
 @pytest.mark.gpu
@@ -69,7 +86,7 @@ def test_sweep_keeps_a_superset_and_reset_clears(gpu, kf, synth, small, fmt):
     k = synth.intrinsics()
     pr = synth.PRESETS["small"]
     vol = kf.Volume(pr.n, pr.extent, pr.trunc, pr.max_weight, fmt)
-    assert not vol.occupancy().any()
+    assert not vol.occupancy().any() and not vol.occupancy_coarse().any()
     integ = kf.VolumeIntegrator(k, integ_params())
     for i in range(5):
         live = kf.DeviceMaps(k, 1, True)
@@ -81,8 +98,9 @@ def test_sweep_keeps_a_superset_and_reset_clears(gpu, kf, synth, small, fmt):
     assert ref.sum() > 100
     assert not (ref & ~occ).any()            # never misses a brick
     assert (occ & ~ref).sum() <= 0.01 * ref.sum()  # and rarely keeps a stale one
+    assert np.array_equal(vol.occupancy_coarse(), coarse_of(occ))
     vol.mark_modified()
     assert np.array_equal(vol.occupancy(), ref)
     kf.General.Instance().InitVoxelArray(vol)
     kf.synchronize()
-    assert not vol.occupancy().any()
+    assert not vol.occupancy().any() and not vol.occupancy_coarse().any()
