@@ -86,16 +86,14 @@ __device__ __forceinline__ bool sample(const RayArgs& a, f3 p, float& out) {
 // every sample whose brick is occupied, then the interpolations.  ok[j] is false
 // outside the voxel-centre domain.
 template <int FMT, int K>
-__device__ __forceinline__ void sample_batch(const RayArgs& a, f3 o, f3 dir, const float* tk, bool* ok,
-                                             float* fk) {
+__device__ __forceinline__ void sample_points(const RayArgs& a, const f3* pts, bool* ok, float* fk) {
   const float lim = (float)(a.n - 1);
   float gx[K], gy[K], gz[K];
   int ix[K], iy[K], iz[K];
   unsigned long long w[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    const float tt = tk[j];
-    const f3 p = f3{o.x + tt * dir.x, o.y + tt * dir.y, o.z + tt * dir.z};
+    const f3 p = pts[j];
     gx[j] = (p.x - a.ox) * a.inv_h - 0.5f;
     gy[j] = (p.y - a.oy) * a.inv_h - 0.5f;
     gz[j] = (p.z - a.oz) * a.inv_h - 0.5f;
@@ -135,6 +133,16 @@ __device__ __forceinline__ void sample_batch(const RayArgs& a, f3 o, f3 dir, con
       fk[j] = lerp(c0, c1, tx);
     }
   }
+}
+
+// K samples along the ray at parameters tk (positions o + t dir, as in sample()).
+template <int FMT, int K>
+__device__ __forceinline__ void sample_batch(const RayArgs& a, f3 o, f3 dir, const float* tk, bool* ok,
+                                             float* fk) {
+  f3 pts[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) pts[j] = f3{o.x + tk[j] * dir.x, o.y + tk[j] * dir.y, o.z + tk[j] * dir.z};
+  sample_points<FMT, K>(a, pts, ok, fk);
 }
 
 template <int FMT>
@@ -191,13 +199,14 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
   auto hit_at = [&](float t, float st, float fp, float fc) {
     const float ts = t + st * (fp / (fp - fc));
     const f3 ph = f3{o.x + ts * dir.x, o.y + ts * dir.y, o.z + ts * dir.z};
-    float gxp, gxm, gyp, gym, gzp, gzm;
-    const bool g = sample<FMT>(a, f3{ph.x + a.h, ph.y, ph.z}, gxp) &&
-                   sample<FMT>(a, f3{ph.x - a.h, ph.y, ph.z}, gxm) &&
-                   sample<FMT>(a, f3{ph.x, ph.y + a.h, ph.z}, gyp) &&
-                   sample<FMT>(a, f3{ph.x, ph.y - a.h, ph.z}, gym) &&
-                   sample<FMT>(a, f3{ph.x, ph.y, ph.z + a.h}, gzp) &&
-                   sample<FMT>(a, f3{ph.x, ph.y, ph.z - a.h}, gzm);
+    // the six central-difference samples, fetched as one batch
+    const f3 q[6] = {f3{ph.x + a.h, ph.y, ph.z}, f3{ph.x - a.h, ph.y, ph.z}, f3{ph.x, ph.y + a.h, ph.z},
+                     f3{ph.x, ph.y - a.h, ph.z}, f3{ph.x, ph.y, ph.z + a.h}, f3{ph.x, ph.y, ph.z - a.h}};
+    bool okq[6];
+    float fq[6];
+    sample_points<FMT, 6>(a, q, okq, fq);
+    const bool g = okq[0] && okq[1] && okq[2] && okq[3] && okq[4] && okq[5];
+    const float gxp = fq[0], gxm = fq[1], gyp = fq[2], gym = fq[3], gzp = fq[4], gzm = fq[5];
     if (g) {
       const f3 gr = f3{gxp - gxm, gyp - gym, gzp - gzm};
       const float gl = sqrtf(dot(gr, gr));

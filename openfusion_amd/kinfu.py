@@ -21,7 +21,7 @@ import os
 import numpy as np
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG, "libkinfu.so")
+LIB_PATH = os.environ.get("KINFU_LIB") or os.path.join(_PKG, "libkinfu.so")
 
 KF_OK = 0
 KF_VOXEL_F32 = 0
