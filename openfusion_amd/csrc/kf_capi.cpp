@@ -396,9 +396,9 @@ extern "C" kf_status kf_tracker_create(const kf_intrinsics* k0, const kf_icp_par
   t->k0 = *k0;
   t->params = *p;
   t->max_blocks = 1024;
-  hipError_t e = hipMalloc(&t->d_partials, (size_t)t->max_blocks * 64 * sizeof(float));
+  hipError_t e = hipMalloc(&t->d_partials, 2 * (size_t)t->max_blocks * 64 * sizeof(float));  // two slot sets
   if (e == hipSuccess) e = hipMalloc(&t->d_ctrl, sizeof(kf_icp_ctrl));
-  if (e == hipSuccess) e = hipMalloc(&t->d_work, sizeof(kf_pose));
+  if (e == hipSuccess) e = hipMalloc(&t->d_work, 2 * sizeof(kf_pose));  // alternating step poses
   if (e == hipSuccess) e = hipMalloc(&t->d_stats_tmp, sizeof(kf_icp_stats));
   if (e == hipSuccess) e = hipMemset(t->d_ctrl, 0, sizeof(kf_icp_ctrl));
   if (e == hipSuccess) e = hipMemset(t->d_stats_tmp, 0, sizeof(kf_icp_stats));

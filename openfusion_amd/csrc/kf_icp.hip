@@ -1,4 +1,4 @@
-// kf_icp.hip — projective point-to-plane ICP on gfx950, two launches per iteration.
+// kf_icp.hip — projective point-to-plane ICP on gfx950, one launch per iteration.
 //
 // Replaces d_find_corresponding / ICP::FindCorresponding (Source/src/ICP.cu:116-262),
 // the host block sum, determinant guard and Eigen LLT of ICP::Iterate_ICP
@@ -14,14 +14,15 @@
 // of (A^T A) x = A^T b the rotation is -x[0..2] (ICP.cpp:162-168).
 //
 // Reduction: every thread accumulates the 28 FP32 terms (21 A^TA, 6 A^Tb, b^2)
-// over a grid-stride run of pixels; waves combine with DPP lane moves (quad
-// permutes, row rotations, row broadcasts) and blocks through LDS.  In
-// compensated mode each term is carried as a (sum, error) pair updated with the
-// error-free TwoSum, so a block partial is accurate to ~2^-48 relative; plain mode uses ordinary FP32 sums.  Blocks write
-// their partials; a one-block launch adds them in FP64 in block order, solves by
-// FP64 Cholesky, composes the Euler-ZYX increment onto the FP64 pose and refreshes
-// the FP32 pose copies, so the next iteration's launch reads the new pose without
-// any host round trip.
+// over its pixels; waves combine with DPP lane moves (reduce-scatter: row
+// mirrors, quad permutes) and blocks through LDS.  In compensated mode each term
+// is carried as a (sum, error) pair updated with the error-free TwoSum, so a block
+// partial is accurate to ~2^-48 relative; plain mode uses ordinary FP32 sums.
+// Each block writes its partial as a slot.  The next launch (the next iteration,
+// or the final one-block launch) adds the slots in FP64 in block order, solves by
+// FP64 Cholesky and composes the Euler-ZYX increment onto the FP64 pose; every
+// block of an iteration repeats that solve for itself, so the pose never makes a
+// round trip through the host and an iteration is a single launch.
 #include "kf_device.hpp"
 #include "kf_internal.hpp"
 #include "kf_pose_math.hpp"
@@ -37,7 +38,7 @@ constexpr double kPivotRelEps = 1e-9;
 constexpr int kMinInliers = 6;
 constexpr int kFinUnroll = 4;       // partial-slot loads in flight per thread in the final sum
 constexpr int kSolveThreads = 1024;  // one block: 64 groups x 16 float4 columns of the slots
-constexpr int kSolveGroups = kSolveThreads / 16;
+constexpr int kStepThreads = 512;    // tracking step: 8 waves per block
 
 struct IcpArgs {
   const float *lvx, *lvy, *lvz, *lnx, *lny, *lnz;  // live level maps (camera frame)
@@ -52,7 +53,6 @@ struct IcpArgs {
   double* sums_out;
   uint8_t* mask;
   int level;
-  int mode;  // 0: track (solve + compose), 1: reduce only
 };
 
 __device__ __forceinline__ void two_sum(float& s, float& c, float v) {
@@ -175,16 +175,18 @@ __device__ __forceinline__ void halve(float* s, float* c, bool upper) {
   }
 }
 
-// Pass 1 of an iteration: association and the block partials (one slot per block).
-template <bool COMP>
-__global__ __launch_bounds__(kThreads) void icp_partials(IcpArgs a) {
-  __shared__ float red_s[4][kPad];
-  __shared__ float red_c[4][kPad];
-  __shared__ int red_n[4][2];
-  if (a.mode == 0 && a.ctrl->stop) return;  // uniform over the grid
-
+// Block partial: this block's share of the association (pixels bid*T + tid + k*nblk*T,
+// two per step with their loads in flight together), the 28 terms per thread,
+// the reduce-scatter wave reduction and the combine over the block's waves,
+// written as one slot: 28 (sum, error) pairs and the inlier / candidate counts.
+template <bool COMP, int WAVES>
+__device__ __forceinline__ void block_partial(const IcpArgs& a, const kf_posef& wc, const kf_posef& cw, int bid,
+                                              int nblk, float* slot) {
+  __shared__ float red_s[WAVES][kPad];
+  __shared__ float red_c[WAVES][kPad];
+  __shared__ int red_n[WAVES][2];
+  constexpr int T = WAVES * kWave;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const kf_posef wc = a.pose->wc, cw = a.pose->cw;
   float s[kPad], c[kPad];  // terms 28..31 stay zero (padding for the halving)
 #pragma unroll
   for (int k = 0; k < kPad; ++k) {
@@ -193,9 +195,8 @@ __global__ __launch_bounds__(kThreads) void icp_partials(IcpArgs a) {
   }
   int n_in = 0, n_cand = 0;
   const int P = a.W * a.H;
-  const int stride = gridDim.x * kThreads;
-  // Two pixels per step (i0, i0 + stride): both pixels' loads are in flight together.
-  for (int i0 = blockIdx.x * kThreads + tid; i0 < P; i0 += 2 * stride) {
+  const int stride = nblk * T;
+  for (int i0 = bid * T + tid; i0 < P; i0 += 2 * stride) {
     const int i1 = i0 + stride;
     const bool has1 = i1 < P;
     const PxModel q0 = load_model(a, i0);
@@ -260,10 +261,9 @@ __global__ __launch_bounds__(kThreads) void icp_partials(IcpArgs a) {
     red_n[wave][1] = n_cand;
   }
   __syncthreads();
-  float* slot = a.partials + (size_t)blockIdx.x * kSlot;
   if (tid < kTerms) {
     float S = red_s[0][tid], C = red_c[0][tid];
-    for (int w = 1; w < 4; ++w) {
+    for (int w = 1; w < WAVES; ++w) {
       if (COMP) {
         float e = 0.0f;
         two_sum(S, e, red_s[w][tid]);
@@ -275,90 +275,92 @@ __global__ __launch_bounds__(kThreads) void icp_partials(IcpArgs a) {
     slot[2 * tid] = S;
     slot[2 * tid + 1] = COMP ? C : 0.0f;
   } else if (tid == kTerms) {
-    const int ni = (red_n[0][0] + red_n[1][0]) + (red_n[2][0] + red_n[3][0]);
-    const int nc = (red_n[0][1] + red_n[1][1]) + (red_n[2][1] + red_n[3][1]);
+    int ni = 0, nc = 0;
+    for (int w = 0; w < WAVES; ++w) {
+      ni += red_n[w][0];
+      nc += red_n[w][1];
+    }
     slot[56] = __int_as_float(ni);
     slot[57] = __int_as_float(nc);
   }
 }
 
-// Pass 2 of an iteration (one block): FP64 sum of the block partials in block
-// order, then the solve and the pose update (or, in reduce mode, the sums).  A
-// separate launch makes the partials visible through the kernel boundary instead
-// of a device-scope fence per block (each fence writes back and invalidates the L2).
-__global__ __launch_bounds__(kSolveThreads) void icp_solve(IcpArgs a, int nb) {
-  __shared__ double fin[kSolveGroups][kTerms + 2];
-  if (a.mode == 0 && a.ctrl->stop) return;  // uniform
+// FP64 sum of nb slots into sum[30] (LDS, valid after the call for every thread).
+// Thread (g, j) reads float4 j of the slots of blocks g, g + G, ... (j < 14: terms
+// 2j, 2j + 1 as (hi, lo) pairs; j = 14: the counts) with kFinUnroll loads in
+// flight; the G group totals of each term are then added in group order.
+template <int THREADS>
+__device__ __forceinline__ void sum_slots(const float* slots, int nb, double* sum) {
+  constexpr int G = THREADS / 16;
+  __shared__ double fin[G][kTerms + 2];
   const int tid = threadIdx.x;
-  // FP64 cross-block sum.  Thread (g, j) reads float4 j of the slots of blocks
-  // g, g + 64, ... (j < 14: terms 2j and 2j + 1 as (hi, lo) pairs; j = 14: the
-  // inlier and candidate counts) with kFinUnroll independent loads in flight,
-  // then the 64 group totals of each term are added in group order.
-  {
-    const int j = tid & 15, g = tid >> 4;
+  const int j = tid & 15, g = tid >> 4;
+  if (j < 15) {
     double a0 = 0.0, a1 = 0.0;
-    if (j < 15) {
-      const float4* base = reinterpret_cast<const float4*>(a.partials) + j;
-      for (int b0 = g; b0 < nb; b0 += kSolveGroups * kFinUnroll) {
-        float4 v[kFinUnroll];
+    const float4* base = reinterpret_cast<const float4*>(slots) + j;
+    for (int b0 = g; b0 < nb; b0 += G * kFinUnroll) {
+      float4 v[kFinUnroll];
 #pragma unroll
-        for (int u = 0; u < kFinUnroll; ++u) {
-          const int bb = b0 + kSolveGroups * u;
-          v[u] = bb < nb ? base[(size_t)bb * (kSlot / 4)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
+      for (int u = 0; u < kFinUnroll; ++u) {
+        const int bb = b0 + G * u;
+        v[u] = bb < nb ? base[(size_t)bb * (kSlot / 4)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
 #pragma unroll
-        for (int u = 0; u < kFinUnroll; ++u) {
-          if (j < 14) {
-            a0 = (a0 + (double)v[u].x) + (double)v[u].y;
-            a1 = (a1 + (double)v[u].z) + (double)v[u].w;
-          } else {
-            a0 = a0 + (double)__float_as_int(v[u].x);
-            a1 = a1 + (double)__float_as_int(v[u].y);
-          }
+      for (int u = 0; u < kFinUnroll; ++u) {
+        if (j < 14) {
+          a0 = (a0 + (double)v[u].x) + (double)v[u].y;
+          a1 = (a1 + (double)v[u].z) + (double)v[u].w;
+        } else {
+          a0 = a0 + (double)__float_as_int(v[u].x);
+          a1 = a1 + (double)__float_as_int(v[u].y);
         }
       }
-      fin[g][2 * j] = a0;
-      fin[g][2 * j + 1] = a1;
     }
+    fin[g][2 * j] = a0;
+    fin[g][2 * j + 1] = a1;
   }
   __syncthreads();
   if (tid < kTerms + 2) {
     double t = fin[0][tid];
-    for (int gg = 1; gg < kSolveGroups; ++gg) t = t + fin[gg][tid];
-    fin[0][tid] = t;
+    for (int gg = 1; gg < G; ++gg) t = t + fin[gg][tid];
+    sum[tid] = t;
   }
   __syncthreads();
-  // Tail: the solve runs on one lane; the Euler sines/cosines (3 lanes), the pose
-  // product (16 lanes) and the FP32 pose fields (24 lanes) run in parallel, each
-  // entry with the same expression as the serial helpers, so the bits are the same.
-  const double* sum = fin[0];
-  if (a.mode == 1) {
-    if (tid < kTerms + 2) a.sums_out[tid] = sum[tid];
-    return;
-  }
-  __shared__ double tx[6], tcs[6], tD[16], tT[16], tN[16];
+}
+
+// Solve (A^T A) x = A^T b from sum[30] and compose the Euler-ZYX increment onto
+// Tin (LDS, 16 doubles): Tout (LDS) and the FP32 fields wc / cw (LDS).  Returns
+// false for a singular system (fewer than kMinInliers inliers or a Cholesky pivot
+// not above 1e-9 x the largest diagonal); then the outputs are untouched.  With
+// `st`, thread 0 writes the level statistics and, when singular, status 1.
+// Cholesky is right-looking and lane-parallel: step j takes the pivot on lane 0,
+// the column below it on lanes j+1..5 and the trailing update on one lane per
+// entry; every entry receives the same subtractions in the same order as the
+// serial left-looking loop (cholesky_solve6), so the factor has the same bits.
+// The substitutions are serial; the sines / cosines (3 lanes), the product (16
+// lanes) and the FP32 fields (24 lanes) are lane-parallel with the expressions of
+// the serial helpers.
+__device__ __forceinline__ bool solve_compose(const double* sum, const double* Tin, kf_icp_stats* st, int level,
+                                              double* Tout, kf_posef* wc, kf_posef* cw) {
+  __shared__ double tx[6], tcs[6], tD[16];
   __shared__ double W[6][6], Lm[6][6];
   __shared__ int tok;
-  // Cholesky, right-looking and lane-parallel: step j takes the pivot on lane 0,
-  // the column below it on lanes j+1..5 and the trailing update on one lane per
-  // entry.  Every entry receives the same subtractions in the same order as the
-  // serial left-looking loop (cholesky_solve6), so the factor has the same bits.
+  const int tid = threadIdx.x;
   if (tid < 36) {
     const int i = tid / 6, k = tid - 6 * (tid / 6);
     const int r = i < k ? i : k, c = i < k ? k : i;
     W[i][k] = sum[r * (11 - r) / 2 + c];
-  } else if (tid >= 64 && tid < 80) {
-    tT[tid - 64] = a.pose->T_wc[tid - 64];
   }
   if (tid == 0) {
     const double inl = sum[28], cand = sum[29];
-    kf_icp_stats* st = a.stats;
     const int ni = (int)inl;
-    st->inliers = ni;
-    st->candidates = (int)cand;
-    st->residual_rms = ni > 0 ? sqrt(sum[27] / inl) : 0.0;
-    st->level_residual[a.level] = st->residual_rms;
-    st->level_inliers[a.level] = ni;
+    if (st) {
+      st->inliers = ni;
+      st->candidates = (int)cand;
+      st->residual_rms = ni > 0 ? sqrt(sum[27] / inl) : 0.0;
+      st->level_residual[level] = st->residual_rms;
+      st->level_inliers[level] = ni;
+    }
     tok = ni >= kMinInliers ? 1 : 0;
   }
   __syncthreads();
@@ -384,7 +386,7 @@ __global__ __launch_bounds__(kSolveThreads) void icp_solve(IcpArgs a, int nb) {
     __syncthreads();
   }
   if (tid == 0) {
-    if (tok) {  // substitutions (serial, same order as cholesky_solve6)
+    if (tok) {
       const double* g = sum + 21;
       double y[6], x[6];
       for (int i = 0; i < 6; ++i) {
@@ -398,65 +400,183 @@ __global__ __launch_bounds__(kSolveThreads) void icp_solve(IcpArgs a, int nb) {
         x[i] = s / Lm[i][i];
       }
       for (int q = 0; q < 6; ++q) tx[q] = x[q];
-    } else {
-      a.ctrl->stop = 1;
-      a.stats->status = 1;
+    } else if (st) {
+      st->status = 1;
     }
   }
   __syncthreads();
-  if (tok) {  // uniform
-    if (tid < 3) {
-      const double ang = -tx[tid];  // rotation increment is the negated solution
-      tcs[2 * tid] = cos(ang);
-      tcs[2 * tid + 1] = sin(ang);
+  if (!tok) return false;  // uniform
+  if (tid < 3) {
+    const double ang = -tx[tid];  // rotation increment is the negated solution
+    tcs[2 * tid] = cos(ang);
+    tcs[2 * tid + 1] = sin(ang);
+  }
+  __syncthreads();
+  if (tid < 16) tD[tid] = euler_zyx_entry(tid, tcs[0], tcs[1], tcs[2], tcs[3], tcs[4], tcs[5], tx[3], tx[4], tx[5]);
+  __syncthreads();
+  if (tid < 16) {
+    const int i = tid >> 2, j = tid & 3;
+    Tout[tid] = (i == 3) ? (j == 3 ? 1.0 : 0.0) : mat4_mul_entry(tD, Tin, i, j);
+  }
+  __syncthreads();
+  if (tid < 12) {  // wc: R (9) and t (3)
+    if (tid < 9) wc->r[tid] = (float)Tout[(tid / 3) * 4 + tid % 3];
+    else wc->t[tid - 9] = (float)Tout[(tid - 9) * 4 + 3];
+  } else if (tid < 24) {  // cw = rigid inverse: R^T and -R^T t
+    const int q = tid - 12;
+    if (q < 9) cw->r[q] = (float)Tout[(q % 3) * 4 + q / 3];
+    else cw->t[q - 9] = (float)rigid_inverse_t(Tout, q - 9);
+  }
+  __syncthreads();
+  return true;
+}
+
+// Copies a pose (thread-parallel, 28 words of 8 bytes).
+__device__ __forceinline__ void copy_pose(const kf_pose* src, kf_pose* dst) {
+  static_assert(sizeof(kf_pose) % 8 == 0, "pose size");
+  const int n = (int)(sizeof(kf_pose) / 8);
+  if ((int)threadIdx.x < n)
+    reinterpret_cast<unsigned long long*>(dst)[threadIdx.x] = reinterpret_cast<const unsigned long long*>(src)[threadIdx.x];
+}
+
+// Reduce mode (kf_icp_reduce), pass 1: the block partials at a fixed pose.
+template <bool COMP>
+__global__ __launch_bounds__(kThreads) void icp_partials(IcpArgs a) {
+  const kf_posef wc = a.pose->wc, cw = a.pose->cw;
+  block_partial<COMP, kThreads / kWave>(a, wc, cw, blockIdx.x, gridDim.x, a.partials + (size_t)blockIdx.x * kSlot);
+}
+
+// Reduce mode, pass 2 (one block): the FP64 sums.
+__global__ __launch_bounds__(kSolveThreads) void icp_sums(IcpArgs a, int nb) {
+  __shared__ double sum[kTerms + 2];
+  sum_slots<kSolveThreads>(a.partials, nb, sum);
+  if ((int)threadIdx.x < kTerms + 2) a.sums_out[threadIdx.x] = sum[threadIdx.x];
+}
+
+// Tracking: one launch per iteration.  Every block first solves the previous
+// iteration from its slots (the same FP64 sums, solve and composition in every
+// block, so all blocks agree bit for bit on the new pose; block 0 publishes it
+// and the statistics), then associates with the new pose and writes its slot.
+// Poses and slots alternate between two buffers, so no block overwrites what
+// another block of the same launch still reads; a kernel boundary publishes
+// them (one launch per iteration instead of a partials and a solve launch).
+struct StepArgs {
+  const kf_pose* pose_in;   // the guess (first step) or the previous step's output
+  kf_pose* pose_out;
+  const float* slots_in;    // previous step's slots (unused for the first step)
+  float* slots_out;
+  int nb_in;                // number of slots in slots_in
+  int level_in;             // level of the previous step
+  int first;
+  const kf_icp_stats* user;  // its skip flag skips the whole call (prior fallback not needed)
+  kf_icp_stats* st;
+};
+
+template <bool COMP>
+__global__ __launch_bounds__(kStepThreads) void icp_step(IcpArgs a, StepArgs S) {
+  __shared__ double sum[kTerms + 2], Tin[16], Tnew[16];
+  __shared__ kf_posef s_wc, s_cw;
+  const int tid = threadIdx.x;
+  if (S.user && S.user->skip) return;  // uniform
+  if (!S.first && a.ctrl->stop) {      // an earlier solve was singular: keep the pose
+    if (blockIdx.x == 0) copy_pose(S.pose_in, S.pose_out);
+    return;
+  }
+  if (S.first) {
+    if (tid < 12) {
+      if (tid < 9) s_wc.r[tid] = S.pose_in->wc.r[tid];
+      else s_wc.t[tid - 9] = S.pose_in->wc.t[tid - 9];
+    } else if (tid < 24) {
+      if (tid < 21) s_cw.r[tid - 12] = S.pose_in->cw.r[tid - 12];
+      else s_cw.t[tid - 21] = S.pose_in->cw.t[tid - 21];
     }
-    __syncthreads();
-    if (tid < 16) tD[tid] = euler_zyx_entry(tid, tcs[0], tcs[1], tcs[2], tcs[3], tcs[4], tcs[5], tx[3], tx[4], tx[5]);
-    __syncthreads();
-    if (tid < 16) {
-      const int i = tid >> 2, j = tid & 3;
-      tN[tid] = (i == 3) ? (j == 3 ? 1.0 : 0.0) : mat4_mul_entry(tD, tT, i, j);
+    if (blockIdx.x == 0) {
+      copy_pose(S.pose_in, S.pose_out);
+      if (tid == 32) {
+        a.ctrl->stop = 0;
+        a.ctrl->executed = 0;
+        kf_icp_stats z = {};
+        *S.st = z;
+      }
     }
+  } else {
+    sum_slots<kStepThreads>(S.slots_in, S.nb_in, sum);
+    if (tid < 16) Tin[tid] = S.pose_in->T_wc[tid];
     __syncthreads();
-    kf_pose* P = a.pose;
-    if (tid < 16) {
-      P->T_wc[tid] = tN[tid];
-    } else if (tid >= 64 && tid < 76) {  // wc: R (9) and t (3)
-      const int q = tid - 64;
-      if (q < 9) P->wc.r[q] = (float)tN[(q / 3) * 4 + q % 3];
-      else P->wc.t[q - 9] = (float)tN[(q - 9) * 4 + 3];
-    } else if (tid >= 128 && tid < 140) {  // cw = rigid inverse: R^T and -R^T t
-      const int q = tid - 128;
-      if (q < 9) P->cw.r[q] = (float)tN[(q % 3) * 4 + q / 3];
-      else P->cw.t[q - 9] = (float)rigid_inverse_t(tN, q - 9);
-    } else if (tid == 192) {
-      a.ctrl->executed = a.ctrl->executed + 1;
+    const bool ok = solve_compose(sum, Tin, blockIdx.x == 0 ? S.st : nullptr, S.level_in, Tnew, &s_wc, &s_cw);
+    if (!ok) {
+      if (blockIdx.x == 0) {
+        copy_pose(S.pose_in, S.pose_out);
+        if (tid == 32) a.ctrl->stop = 1;
+      }
+      return;  // uniform
+    }
+    if (blockIdx.x == 0) {
+      kf_pose* P = S.pose_out;
+      if (tid < 16) P->T_wc[tid] = Tnew[tid];
+      else if (tid >= 64 && tid < 76) {
+        const int q = tid - 64;
+        if (q < 9) P->wc.r[q] = s_wc.r[q]; else P->wc.t[q - 9] = s_wc.t[q - 9];
+      } else if (tid >= 128 && tid < 140) {
+        const int q = tid - 128;
+        if (q < 9) P->cw.r[q] = s_cw.r[q]; else P->cw.t[q - 9] = s_cw.t[q - 9];
+      } else if (tid == 192) {
+        a.ctrl->executed = a.ctrl->executed + 1;
+      }
     }
   }
+  __syncthreads();
+  block_partial<COMP, kStepThreads / kWave>(a, s_wc, s_cw, blockIdx.x, gridDim.x,
+                                            S.slots_out + (size_t)blockIdx.x * kSlot);
 }
 
-
-__global__ void icp_begin(const kf_pose* guess, kf_pose* work, kf_icp_ctrl* ctrl,
-                          const kf_icp_stats* user, kf_icp_stats* st) {
-  if (threadIdx.x != 0) return;
-  const int skip = user ? user->skip : 0;
-  *work = *guess;
-  ctrl->stop = skip ? 2 : 0;
-  ctrl->executed = 0;
-  kf_icp_stats z = {};
-  *st = z;
-}
-
-__global__ void icp_end(const kf_pose* work, kf_pose* out, const kf_icp_ctrl* ctrl,
-                        kf_icp_stats* st, kf_icp_stats* user) {
-  if (threadIdx.x != 0) return;
-  if (ctrl->stop == 2) return;  // skipped call: outputs untouched
-  *out = *work;
-  st->iterations = ctrl->executed;
-  if (user && user != st) {
-    kf_icp_stats c = *st;
-    c.skip = 0;
-    *user = c;
+// End of tracking (one block): solves the last step's slots, writes the output
+// pose and the statistics (iterations = updates applied), copies them to the
+// caller's statistics.  `steps` = 0: no step ran; the guess is the result.
+__global__ __launch_bounds__(kSolveThreads) void icp_finish(IcpArgs a, StepArgs S, int steps, kf_pose* out,
+                                                            kf_icp_stats* user) {
+  __shared__ double sum[kTerms + 2], Tin[16], Tnew[16];
+  __shared__ kf_posef s_wc, s_cw;
+  const int tid = threadIdx.x;
+  if (S.user && S.user->skip) return;  // skipped call: outputs untouched
+  if (steps == 0) {
+    copy_pose(S.pose_in, out);
+    if (tid == 32) {
+      kf_icp_stats z = {};
+      *S.st = z;
+      a.ctrl->stop = 0;
+      a.ctrl->executed = 0;
+    }
+  } else if (a.ctrl->stop) {
+    copy_pose(S.pose_in, out);
+  } else {
+    sum_slots<kSolveThreads>(S.slots_in, S.nb_in, sum);
+    if (tid < 16) Tin[tid] = S.pose_in->T_wc[tid];
+    __syncthreads();
+    if (solve_compose(sum, Tin, S.st, S.level_in, Tnew, &s_wc, &s_cw)) {
+      if (tid < 16) out->T_wc[tid] = Tnew[tid];
+      else if (tid >= 64 && tid < 76) {
+        const int q = tid - 64;
+        if (q < 9) out->wc.r[q] = s_wc.r[q]; else out->wc.t[q - 9] = s_wc.t[q - 9];
+      } else if (tid >= 128 && tid < 140) {
+        const int q = tid - 128;
+        if (q < 9) out->cw.r[q] = s_cw.r[q]; else out->cw.t[q - 9] = s_cw.t[q - 9];
+      } else if (tid == 192) {
+        a.ctrl->executed = a.ctrl->executed + 1;
+      }
+    } else {
+      copy_pose(S.pose_in, out);
+      if (tid == 32) a.ctrl->stop = 1;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    S.st->iterations = a.ctrl->executed;
+    if (user && user != S.st) {
+      kf_icp_stats c = *S.st;
+      c.skip = 0;
+      *user = c;
+    }
   }
 }
 
@@ -498,6 +618,13 @@ IcpArgs level_args(const kf_tracker* t, int l, const kf_frame_maps& live, const 
   return a;
 }
 
+// Tracking steps: 512-thread blocks, 4 / 2 / 1 pixels per thread at levels 0 / 1 / 2+.
+int step_blocks(const kf_tracker* t, int W, int H, int l) {
+  const int ppt = l == 0 ? 4 : (l == 1 ? 2 : 1);
+  int b = div_up(W * H, kStepThreads * ppt);
+  return b < t->max_blocks ? b : t->max_blocks;
+}
+
 int level_blocks(const kf_tracker* t, int W, int H, int l) {
   const int ppt = (l == 0) ? 2 : 1;
   int b = div_up(W * H, kThreads * ppt);
@@ -528,28 +655,38 @@ kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_fr
   const kf_icp_params& p = t->params;
   KF_TRY(check_maps(t, live, model, p.levels));
   kf_icp_stats* st = t->d_stats_tmp;
-  hipLaunchKernelGGL(icp_begin, dim3(1), dim3(64), 0, s, d_guess, t->d_work, t->d_ctrl,
-                     (const kf_icp_stats*)d_stats, st);
-  KF_TRY(check_launch("icp_begin"));
+  float* slots[2] = {t->d_partials, t->d_partials + (size_t)t->max_blocks * kSlot};
+  StepArgs S = {};
+  S.user = d_stats;
+  S.st = st;
+  S.pose_in = d_guess;
+  int step = 0, prev_blocks = 0, prev_level = 0;
+  IcpArgs a = {};
   for (int l = p.levels - 1; l >= 0; --l) {
-    IcpArgs a = level_args(t, l, live, model);
-    a.pose = t->d_work;
-    a.stats = st;
-    a.mode = 0;
-    const int blocks = level_blocks(t, a.W, a.H, l);
-    for (int it = 0; it < p.iterations[l]; ++it) {
+    a = level_args(t, l, live, model);
+    const int blocks = step_blocks(t, a.W, a.H, l);
+    for (int it = 0; it < p.iterations[l]; ++it, ++step) {
+      S.first = step == 0 ? 1 : 0;
+      S.pose_out = &t->d_work[step & 1];
+      S.slots_in = slots[(step + 1) & 1];
+      S.slots_out = slots[step & 1];
+      S.nb_in = prev_blocks;
+      S.level_in = prev_level;
       if (p.compensated)
-        hipLaunchKernelGGL(icp_partials<true>, dim3(blocks), dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL(icp_step<true>, dim3(blocks), dim3(kStepThreads), 0, s, a, S);
       else
-        hipLaunchKernelGGL(icp_partials<false>, dim3(blocks), dim3(kThreads), 0, s, a);
-      KF_TRY(check_launch("icp_partials"));
-      hipLaunchKernelGGL(icp_solve, dim3(1), dim3(kSolveThreads), 0, s, a, blocks);
-      KF_TRY(check_launch("icp_solve"));
+        hipLaunchKernelGGL(icp_step<false>, dim3(blocks), dim3(kStepThreads), 0, s, a, S);
+      KF_TRY(check_launch("icp_step"));
+      S.pose_in = S.pose_out;
+      prev_blocks = blocks;
+      prev_level = l;
     }
   }
-  hipLaunchKernelGGL(icp_end, dim3(1), dim3(64), 0, s, (const kf_pose*)t->d_work, d_out,
-                     (const kf_icp_ctrl*)t->d_ctrl, st, d_stats);
-  return check_launch("icp_end");
+  S.slots_in = slots[(step + 1) & 1];
+  S.nb_in = prev_blocks;
+  S.level_in = prev_level;
+  hipLaunchKernelGGL(icp_finish, dim3(1), dim3(kSolveThreads), 0, s, a, S, step, d_out, d_stats);
+  return check_launch("icp_finish");
 }
 
 kf_status launch_icp_reduce(kf_tracker* t, int level, const kf_frame_maps& live,
@@ -559,19 +696,17 @@ kf_status launch_icp_reduce(kf_tracker* t, int level, const kf_frame_maps& live,
   KF_CHECK_ARG(d_est && d_sums, "pose/sums is NULL");
   KF_TRY(check_maps(t, live, model, level + 1));
   IcpArgs a = level_args(t, level, live, model);
-  a.pose = const_cast<kf_pose*>(d_est);  // read only in reduce mode
-  a.stats = t->d_stats_tmp;
+  a.pose = const_cast<kf_pose*>(d_est);  // read only
   a.sums_out = d_sums;
   a.mask = d_mask;
-  a.mode = 1;
   const int blocks = level_blocks(t, a.W, a.H, level);
   if (t->params.compensated)
     hipLaunchKernelGGL(icp_partials<true>, dim3(blocks), dim3(kThreads), 0, s, a);
   else
     hipLaunchKernelGGL(icp_partials<false>, dim3(blocks), dim3(kThreads), 0, s, a);
   KF_TRY(check_launch("icp_partials(reduce)"));
-  hipLaunchKernelGGL(icp_solve, dim3(1), dim3(kSolveThreads), 0, s, a, blocks);
-  return check_launch("icp_solve(reduce)");
+  hipLaunchKernelGGL(icp_sums, dim3(1), dim3(kSolveThreads), 0, s, a, blocks);
+  return check_launch("icp_sums(reduce)");
 }
 
 kf_status launch_pose_constant_velocity(const kf_pose* d_prev2, const kf_pose* d_prev1,

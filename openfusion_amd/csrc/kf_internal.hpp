@@ -71,9 +71,9 @@ struct kf_tracker {
   kf_intrinsics k0;
   kf_icp_params params;
   int max_blocks = 0;     // partial slots
-  float* d_partials = nullptr;  // max_blocks * 2 * 32 floats (hi, lo)
+  float* d_partials = nullptr;  // two sets of max_blocks slots of 64 floats ((hi, lo) pairs + counts)
   kf_icp_ctrl* d_ctrl = nullptr;
-  kf_pose* d_work = nullptr;     // pose being refined
+  kf_pose* d_work = nullptr;     // two poses: the step outputs alternate between them
   kf_icp_stats* d_stats_tmp = nullptr;  // used when the caller passes none
   double* d_sums_tmp = nullptr;
 };

@@ -217,6 +217,8 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
       }
     }
   };
+  bool found = false;
+  float h_t = 0.0f, h_st = 0.0f, h_fp = 0.0f, h_fc = 0.0f;
   if (active && hit && tnear < tfar) {
     // The march: step_far while the last sample is +1 (free space), step_near
     // otherwise; stop at the first sample < 0, outside the domain, past tfar or
@@ -293,8 +295,12 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
         }
         ++nsteps;
         ++s;
-        if (fk[j] < 0.0f) {
-          hit_at(t, st, fp, fk[j]);
+        if (fk[j] < 0.0f) {  // crossing: refined after the march, for all lanes at once
+          found = true;
+          h_t = t;
+          h_st = st;
+          h_fp = fp;
+          h_fc = fk[j];
           done = true;
           break;
         }
@@ -305,6 +311,9 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
       if (done) break;
     }
   }
+  // After the march, so that the lanes of a wave that found a crossing in
+  // different batches refine it together (one batch of gradient samples).
+  if (found) hit_at(h_t, h_st, h_fp, h_fc);
   if (active) {
     a.out.vx[idx] = vert.x;
     a.out.vy[idx] = vert.y;
