@@ -199,12 +199,13 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
   auto hit_at = [&](float t, float st, float fp, float fc) {
     const float ts = t + st * (fp / (fp - fc));
     const f3 ph = f3{o.x + ts * dir.x, o.y + ts * dir.y, o.z + ts * dir.z};
-    // the six central-difference samples, fetched as one batch
+    // the six central-difference samples, fetched as two batches of three
     const f3 q[6] = {f3{ph.x + a.h, ph.y, ph.z}, f3{ph.x - a.h, ph.y, ph.z}, f3{ph.x, ph.y + a.h, ph.z},
                      f3{ph.x, ph.y - a.h, ph.z}, f3{ph.x, ph.y, ph.z + a.h}, f3{ph.x, ph.y, ph.z - a.h}};
     bool okq[6];
     float fq[6];
-    sample_points<FMT, 6>(a, q, okq, fq);
+    sample_points<FMT, 3>(a, q, okq, fq);
+    sample_points<FMT, 3>(a, q + 3, okq + 3, fq + 3);
     const bool g = okq[0] && okq[1] && okq[2] && okq[3] && okq[4] && okq[5];
     const float gxp = fq[0], gxm = fq[1], gyp = fq[2], gym = fq[3], gzp = fq[4], gzm = fq[5];
     if (g) {
@@ -217,8 +218,6 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
       }
     }
   };
-  bool found = false;
-  float h_t = 0.0f, h_st = 0.0f, h_fp = 0.0f, h_fc = 0.0f;
   if (active && hit && tnear < tfar) {
     // The march: step_far while the last sample is +1 (free space), step_near
     // otherwise; stop at the first sample < 0, outside the domain, past tfar or
@@ -226,8 +225,10 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
     // size stays the same (positions are the sequential ones, t += step) and then
     // consumed in order; a batch ends where the sequential march would change
     // step size or stop, so the result and the step count equal the sequential
-    // march while the dependent-load chain is up to kSpec times shorter.
-    constexpr int kSpec = 4;
+    // march while the dependent-load chain is up to kSpec times shorter.  Two per
+    // batch keeps the kernel near 80 VGPRs (six waves per SIMD); deeper batches
+    // measured slower.
+    constexpr int kSpec = 2;
     float t = tnear, fp;
     bool ok = sample<FMT>(a, f3{o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z}, fp) && fp >= 0.0f;
     int s = 0;
@@ -295,12 +296,8 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
         }
         ++nsteps;
         ++s;
-        if (fk[j] < 0.0f) {  // crossing: refined after the march, for all lanes at once
-          found = true;
-          h_t = t;
-          h_st = st;
-          h_fp = fp;
-          h_fc = fk[j];
+        if (fk[j] < 0.0f) {
+          hit_at(t, st, fp, fk[j]);
           done = true;
           break;
         }
@@ -311,9 +308,6 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
       if (done) break;
     }
   }
-  // After the march, so that the lanes of a wave that found a crossing in
-  // different batches refine it together (one batch of gradient samples).
-  if (found) hit_at(h_t, h_st, h_fp, h_fc);
   if (active) {
     a.out.vx[idx] = vert.x;
     a.out.vy[idx] = vert.y;
