@@ -36,7 +36,7 @@ constexpr int kSlot = 64;    // floats per block partial: 28 (hi,lo) + 2 int cou
 constexpr int kThreads = 256;
 constexpr double kPivotRelEps = 1e-9;
 constexpr int kMinInliers = 6;
-constexpr int kFinUnroll = 4;       // partial-slot loads in flight per thread in the final sum
+constexpr int kFinUnroll = 8;       // partial-slot loads in flight per thread in the final sum
 constexpr int kSolveThreads = 1024;  // one block: 64 groups x 16 float4 columns of the slots
 constexpr int kStepThreads = 512;    // tracking step: 8 waves per block
 
@@ -288,15 +288,16 @@ __device__ __forceinline__ void block_partial(const IcpArgs& a, const kf_posef& 
 // FP64 sum of nb slots into sum[30] (LDS, valid after the call for every thread).
 // Thread (g, j) reads float4 j of the slots of blocks g, g + G, ... (j < 14: terms
 // 2j, 2j + 1 as (hi, lo) pairs; j = 14: the counts) with kFinUnroll loads in
-// flight; the G group totals of each term are then added in group order.
+// flight; the group totals are combined in a fixed order (lane shuffles within a
+// wave, then the waves in order).
 template <int THREADS>
 __device__ __forceinline__ void sum_slots(const float* slots, int nb, double* sum) {
-  constexpr int G = THREADS / 16;
-  __shared__ double fin[G][kTerms + 2];
+  constexpr int G = THREADS / 16, WV = THREADS / kWave;
+  __shared__ double fin[WV][kTerms + 2];
   const int tid = threadIdx.x;
   const int j = tid & 15, g = tid >> 4;
+  double a0 = 0.0, a1 = 0.0;
   if (j < 15) {
-    double a0 = 0.0, a1 = 0.0;
     const float4* base = reinterpret_cast<const float4*>(slots) + j;
     for (int b0 = g; b0 < nb; b0 += G * kFinUnroll) {
       float4 v[kFinUnroll];
@@ -316,13 +317,20 @@ __device__ __forceinline__ void sum_slots(const float* slots, int nb, double* su
         }
       }
     }
-    fin[g][2 * j] = a0;
-    fin[g][2 * j + 1] = a1;
+  }
+  // the four groups of a wave (lanes j, j+16, j+32, j+48), then the waves in order
+  a0 = a0 + __shfl_xor(a0, 16, kWave);
+  a1 = a1 + __shfl_xor(a1, 16, kWave);
+  a0 = a0 + __shfl_xor(a0, 32, kWave);
+  a1 = a1 + __shfl_xor(a1, 32, kWave);
+  if ((tid & 63) < 15) {
+    fin[tid >> 6][2 * j] = a0;
+    fin[tid >> 6][2 * j + 1] = a1;
   }
   __syncthreads();
   if (tid < kTerms + 2) {
     double t = fin[0][tid];
-    for (int gg = 1; gg < G; ++gg) t = t + fin[gg][tid];
+    for (int w = 1; w < WV; ++w) t = t + fin[w][tid];
     sum[tid] = t;
   }
   __syncthreads();
@@ -343,7 +351,7 @@ __device__ __forceinline__ void sum_slots(const float* slots, int nb, double* su
 __device__ __forceinline__ bool solve_compose(const double* sum, const double* Tin, kf_icp_stats* st, int level,
                                               double* Tout, kf_posef* wc, kf_posef* cw) {
   __shared__ double tx[6], tcs[6], tD[16];
-  __shared__ double W[6][6], Lm[6][6];
+  __shared__ double W[6][6], Lm[6][6], Li[6];
   __shared__ int tok;
   const int tid = threadIdx.x;
   if (tid < 36) {
@@ -373,11 +381,17 @@ __device__ __forceinline__ bool solve_compose(const double* sum, const double* T
   for (int j = 0; j < 6 && tok; ++j) {  // tok is read from LDS after a barrier: uniform
     if (tid == 0) {
       const double d = W[j][j];
-      if (!(d > thr)) tok = 0; else Lm[j][j] = sqrt(d);
+      if (!(d > thr)) {
+        tok = 0;
+      } else {
+        const double ljj = sqrt(d);
+        Lm[j][j] = ljj;
+        Li[j] = 1.0 / ljj;
+      }
     }
     __syncthreads();
     if (!tok) break;
-    if (tid > j && tid < 6) Lm[tid][j] = W[tid][j] / Lm[j][j];
+    if (tid > j && tid < 6) Lm[tid][j] = W[tid][j] * Li[j];
     __syncthreads();
     if (tid < 36) {
       const int i = tid / 6, k = tid - 6 * (tid / 6);
@@ -392,12 +406,12 @@ __device__ __forceinline__ bool solve_compose(const double* sum, const double* T
       for (int i = 0; i < 6; ++i) {
         double s = g[i];
         for (int m = 0; m < i; ++m) s = s - Lm[i][m] * y[m];
-        y[i] = s / Lm[i][i];
+        y[i] = s * Li[i];
       }
       for (int i = 5; i >= 0; --i) {
         double s = y[i];
         for (int m = i + 1; m < 6; ++m) s = s - Lm[m][i] * x[m];
-        x[i] = s / Lm[i][i];
+        x[i] = s * Li[i];
       }
       for (int q = 0; q < 6; ++q) tx[q] = x[q];
     } else if (st) {
@@ -408,8 +422,7 @@ __device__ __forceinline__ bool solve_compose(const double* sum, const double* T
   if (!tok) return false;  // uniform
   if (tid < 3) {
     const double ang = -tx[tid];  // rotation increment is the negated solution
-    tcs[2 * tid] = cos(ang);
-    tcs[2 * tid + 1] = sin(ang);
+    small_sincos(ang, &tcs[2 * tid], &tcs[2 * tid + 1]);
   }
   __syncthreads();
   if (tid < 16) tD[tid] = euler_zyx_entry(tid, tcs[0], tcs[1], tcs[2], tcs[3], tcs[4], tcs[5], tx[3], tx[4], tx[5]);

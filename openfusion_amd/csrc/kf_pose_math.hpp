@@ -104,14 +104,20 @@ KF_HD double euler_zyx_entry(int e, double ca, double sa, double cb, double sb, 
   }
 }
 
+KF_HD void small_sincos(double x, double* c, double* s);
+
 KF_HD void euler_zyx(double a, double b, double g, double tx, double ty, double tz, double* D) {
-  const double ca = cos(a), sa = sin(a), cb = cos(b), sb = sin(b), cg = cos(g), sg = sin(g);
+  double ca, sa, cb, sb, cg, sg;
+  small_sincos(a, &ca, &sa);
+  small_sincos(b, &cb, &sb);
+  small_sincos(g, &cg, &sg);
   for (int e = 0; e < 16; ++e) D[e] = euler_zyx_entry(e, ca, sa, cb, sb, cg, sg, tx, ty, tz);
 }
 
 // Solves the 6x6 SPD system A x = g (A given by its 21 upper-triangle entries,
 // row-major i <= j) by Cholesky.  Returns false if a pivot is not above
-// rel_eps * max diagonal (or NaN).
+// rel_eps * max diagonal (or NaN).  Each column divides once (1 / L_jj) and
+// multiplies, so the critical path has six divisions instead of twenty-seven.
 KF_HD bool cholesky_solve6(const double* upper21, const double* g, double rel_eps, double* x) {
   double A[6][6];
   int k = 0;
@@ -124,31 +130,47 @@ KF_HD bool cholesky_solve6(const double* upper21, const double* g, double rel_ep
     }
   for (int i = 0; i < 6; ++i) dmax = A[i][i] > dmax ? A[i][i] : dmax;
   const double thr = rel_eps * dmax;
-  double L[6][6];
+  double L[6][6], inv[6];
   for (int j = 0; j < 6; ++j) {
     double d = A[j][j];
     for (int m = 0; m < j; ++m) d = d - L[j][m] * L[j][m];
     if (!(d > thr)) return false;
     const double ljj = sqrt(d);
     L[j][j] = ljj;
+    inv[j] = 1.0 / ljj;
     for (int i = j + 1; i < 6; ++i) {
       double s = A[i][j];
       for (int m = 0; m < j; ++m) s = s - L[i][m] * L[j][m];
-      L[i][j] = s / ljj;
+      L[i][j] = s * inv[j];
     }
   }
   double y[6];
   for (int i = 0; i < 6; ++i) {
     double s = g[i];
     for (int m = 0; m < i; ++m) s = s - L[i][m] * y[m];
-    y[i] = s / L[i][i];
+    y[i] = s * inv[i];
   }
   for (int i = 5; i >= 0; --i) {
     double s = y[i];
     for (int m = i + 1; m < 6; ++m) s = s - L[m][i] * x[m];
-    x[i] = s / L[i][i];
+    x[i] = s * inv[i];
   }
   return true;
+}
+
+// cos and sin of a small angle: Taylor series through x^14 / x^13 (Horner in x^2)
+// for |x| <= 0.25, where the truncation error is below 1e-20; libm otherwise.
+KF_HD void small_sincos(double x, double* c, double* s) {
+  if (fabs(x) <= 0.25) {
+    const double z = x * x;
+    *s = x * (1.0 + z * (-1.0 / 6.0 + z * (1.0 / 120.0 + z * (-1.0 / 5040.0 + z * (1.0 / 362880.0 +
+             z * (-1.0 / 39916800.0 + z * (1.0 / 6227020800.0)))))));
+    *c = 1.0 + z * (-0.5 + z * (1.0 / 24.0 + z * (-1.0 / 720.0 + z * (1.0 / 40320.0 + z * (-1.0 / 3628800.0 +
+             z * (1.0 / 479001600.0 + z * (-1.0 / 87178291200.0)))))));
+  } else {
+    *c = cos(x);
+    *s = sin(x);
+  }
 }
 
 }  // namespace kf

