@@ -155,13 +155,14 @@ def stage_run(kf, synth, stream, dd, preset: str, W: int, S: int, use_prior: boo
         p.process(dd.ptr + i * fb)
         if i >= W:
             st = p.stats()
-            rows.append((p.stage_ms(), st.updated_voxels, st.raycast_steps))
+            rows.append((p.stage_ms(), st.updated_voxels, st.raycast_steps, p.counters()))
     vstats = p.volume().stats()  # one full read of the volume (also a known-size read for counters)
     p.close()
     ms = np.array([r[0] for r in rows])
     U = np.array([r[1] for r in rows], dtype=np.float64)
     steps = np.array([r[2] for r in rows], dtype=np.float64)
-    return {"ms": ms, "U": U, "steps": steps, "desc": desc, "observed_voxels": float(vstats[0])}
+    cnt = np.array([r[3] for r in rows], dtype=np.float64)
+    return {"ms": ms, "U": U, "steps": steps, "counters": cnt, "desc": desc, "observed_voxels": float(vstats[0])}
 
 
 def kernel_rooflines(kf, d, sr: dict, prior: bool) -> dict:
@@ -181,8 +182,12 @@ def kernel_rooflines(kf, d, sr: dict, prior: bool) -> dict:
     S_bar = float(sr["steps"].mean() / P)
     ray_bytes = P * (S_bar * (Bv // 2) + 24)
     t = float(ms[:, 4].mean())
+    c = sr["counters"].mean(0)
     out["raycast"] = {"kernel": f"raycast<{'FP16' if Bv == 4 else 'FP32'}>", "algorithmic_bytes": ray_bytes,
-                      "avg_launch_ms": t, "mean_steps_per_ray": S_bar}
+                      "avg_launch_ms": t, "mean_steps_per_ray": S_bar,
+                      "per_ray": {"samples_loading_corners": round(float(c[2]) / P, 2),
+                                  "sample_batches": round(float(c[3]) / P, 2),
+                                  "steps_skipped_in_free_space": round(float(c[4]) / P, 2)}}
     icp = d.icp_prior if prior else d.icp
     icp_bytes = 0.0
     for lvl in range(icp.levels):

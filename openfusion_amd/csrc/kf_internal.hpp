@@ -94,7 +94,8 @@ kf_status ensure_occupancy(const kf_volume* v, hipStream_t s);
 constexpr int kOccMaxBricks = 64;  // bricks per axis held in one 64-bit word
 kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_raycast_params& p,
                          const kf_pose* d_pose, kf_frame_maps& model, unsigned long long* d_steps,
-                         hipStream_t s, hipEvent_t after_raycast = nullptr);
+                         hipStream_t s, hipEvent_t after_raycast = nullptr,
+                         unsigned long long* d_diag = nullptr);
 kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_frame_maps& model,
                            const kf_pose* d_guess, kf_pose* d_out, kf_icp_stats* d_stats,
                            hipStream_t s);

@@ -18,6 +18,7 @@ namespace {
 
 enum PoseSlot { kCur = 0, kPrev1 = 1, kPrev2 = 2, kGuess = 3, kGuessPrior = 4, kPriorOut = 5, kSlots = 6 };
 
+constexpr int kCounters = 5;
 constexpr int kWindow = 31;      // accepted residuals kept for the running median
 constexpr int kWindowMin = 5;    // residual test starts once this many are known
 
@@ -53,7 +54,9 @@ struct kf_pipeline {
   bool stage_timing = false;
   hipEvent_t ev[KF_STAGES + 1] = {};
   float stage_ms[KF_STAGES] = {};
-  unsigned long long* d_counters = nullptr;  // [updated voxels, raycast steps] (stage-timing mode)
+  // stage-timing mode: [updated voxels, raycast steps, raycast samples that loaded
+  // corners, raycast batches, raycast steps skipped through free space]
+  unsigned long long* d_counters = nullptr;
   double* d_traj = nullptr;                  // KF_TRAJ_CAPACITY poses (16 doubles each)
   int* d_traj_count = nullptr;
   kf_prior* prior = nullptr;                 // attached learned prior (not owned)
@@ -240,7 +243,7 @@ kf_status record_track_frame(kf_pipeline* p, bool timed) {
   unsigned long long* upd = timed ? &p->d_counters[0] : nullptr;
   unsigned long long* steps = timed ? &p->d_counters[1] : nullptr;
   if (timed) {
-    KF_HIP(hipMemsetAsync(p->d_counters, 0, 2 * sizeof(unsigned long long), s));
+    KF_HIP(hipMemsetAsync(p->d_counters, 0, kCounters * sizeof(unsigned long long), s));
     KF_HIP(hipEventRecord(p->ev[0], s));
   }
   KF_TRY(launch_preprocess(d.intr, d.pre, p->d_depth, p->live, s));
@@ -256,7 +259,8 @@ kf_status record_track_frame(kf_pipeline* p, bool timed) {
   KF_TRY(launch_integrate(p->vol, d.intr, d.integ, p->live.raylen, &p->d_poses[kCur], upd, nullptr, s,
                           timed ? p->ev[3] : nullptr));
   if (timed) KF_HIP(hipEventRecord(p->ev[4], s));
-  KF_TRY(launch_raycast(p->vol, d.intr, d.ray, &p->d_poses[kCur], p->model, steps, s, timed ? p->ev[5] : nullptr));
+  KF_TRY(launch_raycast(p->vol, d.intr, d.ray, &p->d_poses[kCur], p->model, steps, s, timed ? p->ev[5] : nullptr,
+                        timed ? &p->d_counters[2] : nullptr));
   hipLaunchKernelGGL(pose_shift, dim3(1), dim3(64), 0, s, p->d_poses, p->d_traj, p->d_traj_count);
   KF_TRY(check_launch("pose_shift"));
   if (timed) KF_HIP(hipEventRecord(p->ev[6], s));
@@ -384,11 +388,11 @@ extern "C" kf_status kf_pipeline_create(const kf_pipeline_desc* d, kf_stream s, 
     hipError_t e = hipMalloc(&p->d_poses, kSlots * sizeof(kf_pose));
     if (e == hipSuccess) e = hipMalloc(&p->d_stats, 2 * sizeof(kf_icp_stats));
     if (e == hipSuccess) e = hipMalloc(&p->d_depth, (size_t)d->intr.width * d->intr.height * sizeof(uint16_t));
-    if (e == hipSuccess) e = hipMalloc(&p->d_counters, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&p->d_counters, kCounters * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&p->d_traj, (size_t)KF_TRAJ_CAPACITY * 16 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&p->d_traj_count, sizeof(int));
     if (e == hipSuccess) e = hipMemset(p->d_traj_count, 0, sizeof(int));
-    if (e == hipSuccess) e = hipMemset(p->d_counters, 0, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(p->d_counters, 0, kCounters * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(p->d_stats, 0, 2 * sizeof(kf_icp_stats));
     for (int i = 0; i < KF_STAGES + 1 && e == hipSuccess; ++i) e = hipEventCreate(&p->ev[i]);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -492,6 +496,14 @@ extern "C" kf_status kf_pipeline_get_trajectory(kf_pipeline* p, double* T_out, i
     KF_HIP(hipMemcpyAsync(T_out, p->d_traj, (size_t)n * 16 * sizeof(double), hipMemcpyDeviceToHost, p->stream));
     KF_HIP(hipStreamSynchronize(p->stream));
   }
+  return KF_OK;
+}
+
+extern "C" kf_status kf_pipeline_get_counters(kf_pipeline* p, uint64_t out[KF_COUNTERS]) {
+  KF_CHECK_ARG(p && out, "NULL argument");
+  static_assert(KF_COUNTERS == kCounters, "counter count");
+  KF_HIP(hipMemcpyAsync(out, p->d_counters, kCounters * sizeof(uint64_t), hipMemcpyDeviceToHost, p->stream));
+  KF_HIP(hipStreamSynchronize(p->stream));
   return KF_OK;
 }
 

@@ -25,6 +25,7 @@ struct RayArgs {
   const kf_pose* pose;
   kf_level_maps out;
   unsigned long long* steps;
+  unsigned long long* diag;  // optional [samples that loaded corners, batches, skipped steps]
   const unsigned long long* occ;  // brick occupancy map or NULL
   const unsigned* occ2;           // its super-brick level (32^3 voxels)
   int occ_nb, occ_nsb;
@@ -86,7 +87,7 @@ __device__ __forceinline__ bool sample(const RayArgs& a, f3 p, float& out) {
 // every sample whose brick is occupied, then the interpolations.  ok[j] is false
 // outside the voxel-centre domain.
 template <int FMT, int K>
-__device__ __forceinline__ void sample_points(const RayArgs& a, const f3* pts, bool* ok, float* fk) {
+__device__ __forceinline__ int sample_points(const RayArgs& a, const f3* pts, bool* ok, float* fk) {
   const float lim = (float)(a.n - 1);
   float gx[K], gy[K], gz[K];
   int ix[K], iy[K], iz[K];
@@ -122,9 +123,11 @@ __device__ __forceinline__ void sample_points(const RayArgs& a, const f3* pts, b
       c[j][7] = vox_f<FMT>(a.vox, b + sx + sy + 1);
     }
   }
+  int loads = 0;
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     fk[j] = 1.0f;
+    loads += need[j] ? 1 : 0;
     if (need[j]) {
       const float tx = gx[j] - (float)ix[j], ty = gy[j] - (float)iy[j], tz = gz[j] - (float)iz[j];
       const float c00 = lerp(c[j][0], c[j][1], tz), c01 = lerp(c[j][2], c[j][3], tz);
@@ -133,16 +136,17 @@ __device__ __forceinline__ void sample_points(const RayArgs& a, const f3* pts, b
       fk[j] = lerp(c0, c1, tx);
     }
   }
+  return loads;
 }
 
 // K samples along the ray at parameters tk (positions o + t dir, as in sample()).
 template <int FMT, int K>
-__device__ __forceinline__ void sample_batch(const RayArgs& a, f3 o, f3 dir, const float* tk, bool* ok,
-                                             float* fk) {
+__device__ __forceinline__ int sample_batch(const RayArgs& a, f3 o, f3 dir, const float* tk, bool* ok,
+                                            float* fk) {
   f3 pts[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) pts[j] = f3{o.x + tk[j] * dir.x, o.y + tk[j] * dir.y, o.z + tk[j] * dir.z};
-  sample_points<FMT, K>(a, pts, ok, fk);
+  return sample_points<FMT, K>(a, pts, ok, fk);
 }
 
 template <int FMT>
@@ -194,6 +198,7 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
   }
   f3 vert = f3{nanf_(), nanf_(), nanf_()}, nrm = vert;
   int nsteps = 0;
+  int n_loads = 0, n_batches = 0, n_skipped = 0;  // diagnostics (stage-timing mode)
   // Zero crossing between t (sample fp >= 0) and t + st (sample fc < 0): linear
   // refinement and trilinear central-difference normal.
   auto hit_at = [&](float t, float st, float fp, float fc) {
@@ -269,6 +274,7 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
         t_ = tt;
         s_ += k;
         nsteps += k;
+        n_skipped += k;
       }
     };
     while (ok && s < a.max_steps) {
@@ -283,7 +289,8 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
         tt = tt + st;
         tk[j] = tt;
       }
-      sample_batch<FMT, kSpec>(a, o, dir, tk, vk, fk);
+      n_loads += sample_batch<FMT, kSpec>(a, o, dir, tk, vk, fk);
+      ++n_batches;
 #pragma unroll
       for (int j = 0; j < kSpec; ++j) vk[j] = vk[j] && tk[j] <= tfar;
       bool done = false;
@@ -315,6 +322,21 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
     a.out.nx[idx] = nrm.x;
     a.out.ny[idx] = nrm.y;
     a.out.nz[idx] = nrm.z;
+  }
+  if (a.diag) {  // diagnostics: one device atomic per block and counter
+    __shared__ int dred[3][4];
+    const int v0 = wave_sum_i_lane63(n_loads), v1 = wave_sum_i_lane63(n_batches), v2 = wave_sum_i_lane63(n_skipped);
+    if (lane == 63) {
+      dred[0][wave] = v0;
+      dred[1][wave] = v1;
+      dred[2][wave] = v2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      const int q = threadIdx.x;
+      const int b = (dred[q][0] + dred[q][1]) + (dred[q][2] + dred[q][3]);
+      if (b) atomicAdd(&a.diag[q], (unsigned long long)b);
+    }
   }
   if (a.steps) {  // one device atomic per block (uniform branch)
     __shared__ int red[4];
@@ -359,7 +381,7 @@ __global__ __launch_bounds__(256) void subsample_maps(SubArgs s) {
 
 kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_raycast_params& p,
                          const kf_pose* d_pose, kf_frame_maps& model, unsigned long long* d_steps,
-                         hipStream_t s, hipEvent_t after_raycast) {
+                         hipStream_t s, hipEvent_t after_raycast, unsigned long long* d_diag) {
   KF_CHECK_ARG(d_pose != nullptr, "pose is NULL");
   KF_CHECK_ARG(model.levels >= 1 && model.levels <= KF_MAX_LEVELS, "model levels must be 1..3");
   KF_CHECK_ARG(p.step_far > 0.0f && p.step_near > 0.0f && p.max_steps > 0, "bad march steps");
@@ -375,6 +397,7 @@ kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_r
   a.pose = d_pose;
   a.out = model.level[0];
   a.steps = d_steps;
+  a.diag = d_diag;
   KF_TRY(ensure_occupancy(v, s));
   a.occ = v->d_occ;
   a.occ2 = v->d_occ2;

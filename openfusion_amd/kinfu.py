@@ -185,6 +185,7 @@ _PROTOS = {
     "kf_pipeline_get_maps": (C.c_int, [_VP, C.POINTER(FrameMaps), C.POINTER(FrameMaps)]),
     "kf_pipeline_get_stream": (C.c_int, [_VP, C.POINTER(_VP)]),
     "kf_pipeline_set_stage_timing": (C.c_int, [_VP, C.c_int32]),
+    "kf_pipeline_get_counters": (C.c_int, [_VP, _VP]),
     "kf_pipeline_get_stage_ms": (C.c_int, [_VP, C.POINTER(C.c_float)]),
 }
 
@@ -762,6 +763,12 @@ class Pipeline:
 
     def set_stage_timing(self, on: bool) -> None:
         check(lib().kf_pipeline_set_stage_timing(self.handle, int(on)), "set_stage_timing")
+
+    def counters(self) -> np.ndarray:
+        """Stage-timing counters of the last frame (see KF_COUNTERS in kinfu.h)."""
+        out = np.zeros(5, dtype=np.uint64)
+        check(lib().kf_pipeline_get_counters(self.handle, out.ctypes.data), "get_counters")
+        return out
 
     def stage_ms(self) -> np.ndarray:
         ms = (C.c_float * STAGES)()
