@@ -88,6 +88,12 @@ struct GemmArgs {
   long ldc;
   float* Cpart;
   int M, N, K, kps, accumulate;
+  // Epilogue (no split-K): v = acc [+ bias[n], ReLU] [+ addend[m * N + n]]; C = v and,
+  // with Cb, the BF16 copy (RNE) at the same position.
+  const float* bias;
+  int relu;
+  const float* addend;
+  uint16_t* Cb;
 };
 
 // Each 256-thread block computes a 32x32 tile of C (four 16x16 wave tiles on
@@ -202,7 +208,14 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
       g.Cpart[((long)blockIdx.z * g.M + cm) * g.N + cn] = acc[j];
     } else {
       float* c = g.C + (long)cm * g.ldc + cn;
-      *c = g.accumulate ? *c + acc[j] : acc[j];
+      float v = g.accumulate ? *c + acc[j] : acc[j];
+      if (g.bias) {
+        v = v + g.bias[cn];
+        if (g.relu) v = fmaxf(v, 0.0f);
+      }
+      if (g.addend) v = v + g.addend[(long)cm * g.N + cn];
+      *c = v;
+      if (g.Cb) g.Cb[(long)cm * g.ldc + cn] = bf16_rne(v);
     }
   }
 }
@@ -285,37 +298,26 @@ __global__ void col2im_relu(const float* dcol, const float* X, int layer, int ba
   dX[i] = X[i] > 0.0f ? s : 0.0f;
 }
 
-__global__ void bias_relu(float* Y, const float* b, long M, int C, int relu) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M * C) return;
-  const float v = Y[i] + b[i % C];
-  Y[i] = relu ? fmaxf(v, 0.0f) : v;
-}
-
-// Output skip connection: O[n][j] += prev twist[n][j] (the network predicts the
-// change of motion; zero output = constant velocity).
-__global__ void add_prev(float* O, const float* prev, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) O[i] = O[i] + prev[i];
-}
-
 __global__ void to_bf16(const float* x, uint16_t* y, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = bf16_rne(x[i]);
 }
 
 // feat[n][0..127] = mean over the 80 pooled pixels of Y4; [128..133] = prev twist.
-__global__ void pool_concat(const float* Y4, const float* prev, int batch, float* X5) {
+__global__ void pool_concat(const float* Y4, const float* prev, int batch, float* X5, uint16_t* X5b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= batch * kFc1In) return;
   const int n = i / kFc1In, c = i - n * kFc1In;
+  float v;
   if (c < kFeat) {
     float s = 0.0f;
     for (int p = 0; p < kPool; ++p) s = s + Y4[((long)n * kPool + p) * kFeat + c];
-    X5[i] = s * (1.0f / (float)kPool);
+    v = s * (1.0f / (float)kPool);
   } else {
-    X5[i] = prev[n * kTwist + (c - kFeat)];
+    v = prev[n * kTwist + (c - kFeat)];
   }
+  X5[i] = v;
+  X5b[i] = bf16_rne(v);
 }
 
 // dY4[n][p][c] = dX5[n][c] / 80 where Y4 > 0.
@@ -420,9 +422,21 @@ namespace {
 
 constexpr int kSplitMaxK = 4096;  // pixels per split in weight-gradient GEMMs
 
+struct Epilogue {
+  const float* bias = nullptr;
+  int relu = 0;
+  const float* addend = nullptr;
+  uint16_t* Cb = nullptr;
+};
+
 kf_status gemm(hipStream_t s, const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, float* C,
-               long ldc, int M, int N, int K, bool accumulate, float* part = nullptr, long part_floats = 0) {
+               long ldc, int M, int N, int K, bool accumulate, float* part = nullptr, long part_floats = 0,
+               const Epilogue& ep = Epilogue()) {
   GemmArgs g;
+  g.bias = ep.bias;
+  g.relu = ep.relu;
+  g.addend = ep.addend;
+  g.Cb = ep.Cb;
   g.A = A;
   g.sam = sam;
   g.sak = sak;
@@ -447,7 +461,7 @@ kf_status gemm(hipStream_t s, const uint16_t* A, long sam, long sak, const uint1
   hipLaunchKernelGGL(gemm_bf16, grid, dim3(256), 0, s, g);
   KF_TRY(check_launch("gemm_bf16"));
   if (splits > 1) {
-    KF_CHECK_ARG(!accumulate, "split-K GEMM cannot accumulate");
+    KF_CHECK_ARG(!accumulate && !ep.bias && !ep.addend && !ep.Cb, "split-K GEMM has no epilogue");
     const long MN = (long)M * N;
     hipLaunchKernelGGL(splitk_reduce, dim3(blocks_for(MN)), dim3(256), 0, s, (const float*)part, splits, MN, N, C,
                        ldc);
@@ -467,7 +481,9 @@ kf_status sync_weights(kf_prior* P, hipStream_t s) {
   return check_launch("to_bf16(weights)");
 }
 
-kf_status forward(kf_prior* P, const float* frames, const int32_t* pairs, const float* prev, int B, hipStream_t s) {
+// out: where the twists go (P->O when NULL; training reads P->O).
+kf_status forward(kf_prior* P, const float* frames, const int32_t* pairs, const float* prev, int B, hipStream_t s,
+                  float* out = nullptr) {
   const ParamLayout& L = layout();
   for (int l = 0; l < kConvs; ++l) {
     const ConvShape& c = conv(l);
@@ -480,25 +496,32 @@ kf_status forward(kf_prior* P, const float* frames, const int32_t* pairs, const 
                          P->col[l]);
     KF_TRY(check_launch("im2col"));
     // Y[m][co] = sum_k col[m][k] W[co][k]; the layer-1 padding columns are zero in col.
-    KF_TRY(gemm(s, P->col[l], kcp(l), 1, P->wbf + L.w[l], 1, c.kc, P->Y[l], c.cout, (int)M, c.cout, c.kc, false));
-    hipLaunchKernelGGL(bias_relu, dim3(blocks_for(M * c.cout)), dim3(256), 0, s, P->Y[l],
-                       (const float*)P->params + L.b[l], M, c.cout, 1);
-    KF_TRY(check_launch("bias_relu"));
+    // Y = ReLU(col W^T + b): bias and ReLU in the GEMM epilogue
+    Epilogue ep;
+    ep.bias = (const float*)P->params + L.b[l];
+    ep.relu = 1;
+    KF_TRY(gemm(s, P->col[l], kcp(l), 1, P->wbf + L.w[l], 1, c.kc, P->Y[l], c.cout, (int)M, c.cout, c.kc, false,
+                nullptr, 0, ep));
   }
   hipLaunchKernelGGL(pool_concat, dim3(blocks_for((long)B * kFc1In)), dim3(256), 0, s, (const float*)P->Y[3], prev, B,
-                     P->X5);
-  hipLaunchKernelGGL(to_bf16, dim3(blocks_for((long)B * kFc1In)), dim3(256), 0, s, (const float*)P->X5, P->X5b,
-                     (long)B * kFc1In);
-  KF_TRY(gemm(s, P->X5b, kFc1In, 1, P->wbf + L.w[4], 1, kFc1In, P->Hd, kHidden, B, kHidden, kFc1In, false));
-  hipLaunchKernelGGL(bias_relu, dim3(blocks_for((long)B * kHidden)), dim3(256), 0, s, P->Hd,
-                     (const float*)P->params + L.b[4], (long)B, kHidden, 1);
-  hipLaunchKernelGGL(to_bf16, dim3(blocks_for((long)B * kHidden)), dim3(256), 0, s, (const float*)P->Hd, P->Hb,
-                     (long)B * kHidden);
-  KF_TRY(gemm(s, P->Hb, kHidden, 1, P->wbf + L.w[5], 1, kHidden, P->O, kTwist, B, kTwist, kHidden, false));
-  hipLaunchKernelGGL(bias_relu, dim3(blocks_for((long)B * kTwist)), dim3(256), 0, s, P->O,
-                     (const float*)P->params + L.b[5], (long)B, kTwist, 0);
-  hipLaunchKernelGGL(add_prev, dim3(blocks_for((long)B * kTwist)), dim3(256), 0, s, P->O, prev, B * kTwist);
-  return check_launch("prior forward");
+                     P->X5, P->X5b);
+  KF_TRY(check_launch("pool_concat"));
+  {  // FC1: H = ReLU(X5 W5^T + b5), also as BF16 for FC2
+    Epilogue ep;
+    ep.bias = (const float*)P->params + L.b[4];
+    ep.relu = 1;
+    ep.Cb = P->Hb;
+    KF_TRY(gemm(s, P->X5b, kFc1In, 1, P->wbf + L.w[4], 1, kFc1In, P->Hd, kHidden, B, kHidden, kFc1In, false, nullptr,
+                0, ep));
+  }
+  {  // FC2 plus the skip connection: O = (H W6^T + b6) + previous twist
+    Epilogue ep;
+    ep.bias = (const float*)P->params + L.b[5];
+    ep.addend = prev;
+    KF_TRY(gemm(s, P->Hb, kHidden, 1, P->wbf + L.w[5], 1, kHidden, out ? out : P->O, kTwist, B, kTwist, kHidden,
+                false, nullptr, 0, ep));
+  }
+  return KF_OK;
 }
 
 kf_status backward(kf_prior* P, const float* target, int B, float* loss, hipStream_t s) {
@@ -674,9 +697,7 @@ extern "C" kf_status kf_prior_forward(kf_prior* P, const float* frames, const in
                                       int32_t B, float* d_out, kf_stream s) {
   KF_TRY(check_batch(P, frames, pairs, prev, B));
   KF_CHECK_ARG(d_out != nullptr, "out is NULL");
-  KF_TRY(forward(P, frames, pairs, prev, B, S(s)));
-  KF_HIP(hipMemcpyAsync(d_out, P->O, (size_t)B * kTwist * 4, hipMemcpyDeviceToDevice, S(s)));
-  return KF_OK;
+  return forward(P, frames, pairs, prev, B, S(s), d_out);
 }
 
 extern "C" kf_status kf_prior_gradients(kf_prior* P, const float* frames, const int32_t* pairs, const float* prev,
