@@ -87,7 +87,8 @@ __device__ __forceinline__ bool sample(const RayArgs& a, f3 p, float& out) {
 // every sample whose brick is occupied, then the interpolations.  ok[j] is false
 // outside the voxel-centre domain.
 template <int FMT, int K>
-__device__ __forceinline__ int sample_points(const RayArgs& a, const f3* pts, bool* ok, float* fk) {
+__device__ __forceinline__ int sample_points(const RayArgs& a, const f3* pts, bool* ok, float* fk,
+                                             bool* clear = nullptr) {
   const float lim = (float)(a.n - 1);
   float gx[K], gy[K], gz[K];
   int ix[K], iy[K], iz[K];
@@ -111,6 +112,7 @@ __device__ __forceinline__ int sample_points(const RayArgs& a, const f3* pts, bo
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     need[j] = ok[j] && ((w[j] >> (iz[j] >> 3)) & 1ull);
+    if (clear) clear[j] = ok[j] && !need[j] && a.occ;
     if (need[j]) {
       const size_t b = ((size_t)ix[j] * n + (size_t)iy[j]) * n + (size_t)iz[j];
       c[j][0] = vox_f<FMT>(a.vox, b);
@@ -142,11 +144,11 @@ __device__ __forceinline__ int sample_points(const RayArgs& a, const f3* pts, bo
 // K samples along the ray at parameters tk (positions o + t dir, as in sample()).
 template <int FMT, int K>
 __device__ __forceinline__ int sample_batch(const RayArgs& a, f3 o, f3 dir, const float* tk, bool* ok,
-                                            float* fk) {
+                                            float* fk, bool* clear) {
   f3 pts[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) pts[j] = f3{o.x + tk[j] * dir.x, o.y + tk[j] * dir.y, o.z + tk[j] * dir.z};
-  return sample_points<FMT, K>(a, pts, ok, fk);
+  return sample_points<FMT, K>(a, pts, ok, fk, clear);
 }
 
 template <int FMT>
@@ -244,16 +246,25 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
     // still in the super-brick (checked with the exact sample() arithmetic) every
     // intermediate one is too, and each of their samples is exactly 1.  The step
     // count, the position and the value are those of the step-by-step march.
+    // Free-space skip.  From a far-mode position t (sample 1) whose base voxel lies
+    // in a clear box — its 32^3 super-brick when that is clear, else its brick when
+    // the brick is clear — advance k far steps at once: the k positions are the
+    // sequential ones (t += step, k additions), and the grid coordinate of a
+    // position is a monotone function of t on each axis, so if the last one is
+    // still in the box (checked with the exact sample() arithmetic) every
+    // intermediate one is too, and each of their samples is exactly 1.  The step
+    // count, the position and the value are those of the step-by-step march.
     auto skip_free = [&](float& t_, int& s_) {
       const float gx = ((o.x + t_ * dir.x) - a.ox) * a.inv_h - 0.5f;
       const float gy = ((o.y + t_ * dir.y) - a.oy) * a.inv_h - 0.5f;
       const float gz = ((o.z + t_ * dir.z) - a.oz) * a.inv_h - 0.5f;
-      const int sx = (int)gx >> 5, sy = (int)gy >> 5, sz = (int)gz >> 5;
-      if ((s_super[sx * a.occ_nsb + sy] >> sz) & 1u) return;
+      const int ix = (int)gx, iy = (int)gy, iz = (int)gz;
+      const bool super_clear = !((s_super[(ix >> 5) * a.occ_nsb + (iy >> 5)] >> (iz >> 5)) & 1u);
+      const int sh = super_clear ? 5 : 3;  // box: super-brick (32) or brick (8)
       const float lim = (float)(a.n - 1);
-      const float lx = (float)(32 * sx), hx = fminf((float)(32 * sx + 32), lim);
-      const float ly = (float)(32 * sy), hy = fminf((float)(32 * sy + 32), lim);
-      const float lz = (float)(32 * sz), hz = fminf((float)(32 * sz + 32), lim);
+      const float lx = (float)((ix >> sh) << sh), hx = fminf((float)(((ix >> sh) + 1) << sh), lim);
+      const float ly = (float)((iy >> sh) << sh), hy = fminf((float)(((iy >> sh) + 1) << sh), lim);
+      const float lz = (float)((iz >> sh) << sh), hz = fminf((float)(((iz >> sh) + 1) << sh), lim);
       // approximate exit parameter of the box (grid value G at world (G + 0.5) h + origin)
       float te = tfar;
       if (dir.x > 0.0f) te = fminf(te, ((hx + 0.5f) * a.h + a.ox - o.x) / dir.x);
@@ -277,19 +288,20 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
         n_skipped += k;
       }
     };
+    bool last_clear = false;  // the sample at t has its base voxel in a clear brick
     while (ok && s < a.max_steps) {
-      if (nsw && fp >= 1.0f) skip_free(t, s);
+      if (nsw && last_clear && fp >= 1.0f) skip_free(t, s);
       const bool far = fp >= 1.0f;
       const float st = far ? a.step_far_m : a.step_near_m;
       float tk[kSpec], fk[kSpec];
-      bool vk[kSpec];
+      bool vk[kSpec], ck[kSpec];
       float tt = t;
 #pragma unroll
       for (int j = 0; j < kSpec; ++j) {
         tt = tt + st;
         tk[j] = tt;
       }
-      n_loads += sample_batch<FMT, kSpec>(a, o, dir, tk, vk, fk);
+      n_loads += sample_batch<FMT, kSpec>(a, o, dir, tk, vk, fk, ck);
       ++n_batches;
 #pragma unroll
       for (int j = 0; j < kSpec; ++j) vk[j] = vk[j] && tk[j] <= tfar;
@@ -310,6 +322,7 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
         }
         t = tk[j];
         fp = fk[j];
+        last_clear = ck[j];
         if ((fp >= 1.0f) != far) break;  // the next step has the other size
       }
       if (done) break;
