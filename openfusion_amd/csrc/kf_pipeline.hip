@@ -18,7 +18,7 @@ namespace {
 
 enum PoseSlot { kCur = 0, kPrev1 = 1, kPrev2 = 2, kGuess = 3, kGuessPrior = 4, kPriorOut = 5, kSlots = 6 };
 
-constexpr int kCounters = 5;
+constexpr int kCounters = 6;
 constexpr int kWindow = 31;      // accepted residuals kept for the running median
 constexpr int kWindowMin = 5;    // residual test starts once this many are known
 
@@ -55,7 +55,8 @@ struct kf_pipeline {
   hipEvent_t ev[KF_STAGES + 1] = {};
   float stage_ms[KF_STAGES] = {};
   // stage-timing mode: [updated voxels, raycast steps, raycast samples that loaded
-  // corners, raycast batches, raycast steps skipped through free space]
+  // corners, raycast batches, raycast steps skipped through free space, sum over
+  // wavefronts of 64 x the most batches of one of its rays]
   unsigned long long* d_counters = nullptr;
   double* d_traj = nullptr;                  // KF_TRAJ_CAPACITY poses (16 doubles each)
   int* d_traj_count = nullptr;

@@ -25,7 +25,7 @@ struct RayArgs {
   const kf_pose* pose;
   kf_level_maps out;
   unsigned long long* steps;
-  unsigned long long* diag;  // optional [samples that loaded corners, batches, skipped steps]
+  unsigned long long* diag;  // optional [samples that loaded corners, batches, skipped steps, 64 x wave max batches]
   const unsigned long long* occ;  // brick occupancy map or NULL
   const unsigned* occ2;           // its super-brick level (32^3 voxels)
   int occ_nb, occ_nsb;
@@ -337,15 +337,18 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
     a.out.nz[idx] = nrm.z;
   }
   if (a.diag) {  // diagnostics: one device atomic per block and counter
-    __shared__ int dred[3][4];
+    __shared__ int dred[4][4];
     const int v0 = wave_sum_i_lane63(n_loads), v1 = wave_sum_i_lane63(n_batches), v2 = wave_sum_i_lane63(n_skipped);
+    int mx = n_batches;  // the wave runs as long as its longest ray
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, kWave));
     if (lane == 63) {
       dred[0][wave] = v0;
       dred[1][wave] = v1;
       dred[2][wave] = v2;
+      dred[3][wave] = mx * kWave;
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
+    if (threadIdx.x < 4) {
       const int q = threadIdx.x;
       const int b = (dred[q][0] + dred[q][1]) + (dred[q][2] + dred[q][3]);
       if (b) atomicAdd(&a.diag[q], (unsigned long long)b);
