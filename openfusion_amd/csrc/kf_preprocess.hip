@@ -6,7 +6,8 @@
 // Maps are kept in the camera frame; the tracker applies the pose itself.
 //
 //   depth_to_l0   : uint16 -> metres, range gate, raylen = |K^-1 [x,y,1]| * d,
-//                   bilateral filter (LDS tile with halo)            [per L0 pixel]
+//                   bilateral filter (LDS tile with halo, spatial and range
+//                   weights from per-block LDS tables)                [per L0 pixel]
 //   pyr_down      : 2x2 block average gated by |d - centre| < gate   [per coarse pixel]
 //   vertex_normal : v = d * K^-1 [x,y,1];  n = normalize((v(x,y+1)-v) x (v(x+1,y)-v))
 //                   (points towards the camera), all levels in one launch
@@ -31,8 +32,16 @@ struct PreArgs {
   float inv2ss, inv2sr;  // 0.5/sigma_s^2, 0.5/sigma_r^2
 };
 
+// Bilateral weight of a neighbour: exp(-|dp|^2 / 2 sigma_s^2) * exp(-dr^2 / 2 sigma_r^2)
+// with dr = |u_q - u_c| * scale (the integer depth-code difference in metres).  Both
+// factors come from per-block LDS tables (spatial: one per offset; range: one per
+// code difference below kRangeLut), each entry evaluated with exp_neg.
+constexpr int kRangeLut = 512;
+
 __global__ __launch_bounds__(256) void depth_to_l0(PreArgs a) {
-  __shared__ float tile[kHalo * kHalo];
+  __shared__ int tile[kHalo * kHalo];  // depth codes, 0 = invalid
+  __shared__ float ws[(2 * kMaxRadius + 1) * (2 * kMaxRadius + 1)];
+  __shared__ float wr[kRangeLut];
   const int R = a.radius;
   const int span = kTile + 2 * R;
   const int bx = blockIdx.x * kTile, by = blockIdx.y * kTile;
@@ -40,23 +49,34 @@ __global__ __launch_bounds__(256) void depth_to_l0(PreArgs a) {
   for (int i = tid; i < span * span; i += kTile * kTile) {
     int ty = i / span, tx = i - ty * span;
     int gx = bx + tx - R, gy = by + ty - R;
-    float d = 0.0f;
+    int u = 0;
     if (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H) {
-      d = (float)a.depth[gy * a.W + gx] * a.scale;
-      if (!(d >= a.dmin && d <= a.dmax)) d = 0.0f;
+      u = a.depth[gy * a.W + gx];
+      const float d = (float)u * a.scale;
+      if (!(d >= a.dmin && d <= a.dmax)) u = 0;
     }
-    tile[ty * span + tx] = d;
+    tile[ty * span + tx] = u;
+  }
+  const int nsp = (2 * R + 1) * (2 * R + 1);
+  if (tid < nsp) {
+    const int dy = tid / (2 * R + 1) - R, dx = tid % (2 * R + 1) - R;
+    ws[tid] = exp_neg((float)(dx * dx + dy * dy) * a.inv2ss);
+  }
+  for (int i = tid; i < kRangeLut; i += kTile * kTile) {
+    const float rm = (float)i * a.scale;
+    wr[i] = exp_neg((rm * rm) * a.inv2sr);
   }
   __syncthreads();
   const int x = bx + threadIdx.x, y = by + threadIdx.y;
   if (x >= a.W || y >= a.H) return;
-  const float dc = tile[(threadIdx.y + R) * span + threadIdx.x + R];
+  const int uc = tile[(threadIdx.y + R) * span + threadIdx.x + R];
   const int idx = y * a.W + x;
-  if (dc == 0.0f) {
+  if (uc == 0) {
     a.raylen[idx] = 0.0f;
     a.out[idx] = 0.0f;
     return;
   }
+  const float dc = (float)uc * a.scale;
   const float xn = ((float)x - a.cx) / a.fx;
   const float yn = ((float)y - a.cy) / a.fy;
   const float lam = sqrtf((xn * xn + yn * yn) + 1.0f);
@@ -69,11 +89,18 @@ __global__ __launch_bounds__(256) void depth_to_l0(PreArgs a) {
     for (int dx = -R; dx <= R; ++dx) {
       const int qx = x + dx;
       if (qx < 0 || qx >= a.W) continue;
-      const float dq = tile[(threadIdx.y + R + dy) * span + threadIdx.x + R + dx];
-      if (dq == 0.0f) continue;
-      const float diff = dq - dc;
-      const float arg = (float)(dx * dx + dy * dy) * a.inv2ss + (diff * diff) * a.inv2sr;
-      const float w = exp_neg(arg);
+      const int uq = tile[(threadIdx.y + R + dy) * span + threadIdx.x + R + dx];
+      if (uq == 0) continue;
+      const int du = uq > uc ? uq - uc : uc - uq;
+      float w_r;
+      if (du < kRangeLut) {
+        w_r = wr[du];
+      } else {
+        const float rm = (float)du * a.scale;
+        w_r = exp_neg((rm * rm) * a.inv2sr);
+      }
+      const float w = ws[(dy + R) * (2 * R + 1) + (dx + R)] * w_r;
+      const float dq = (float)uq * a.scale;
       sw = sw + w;
       swd = swd + w * dq;
     }

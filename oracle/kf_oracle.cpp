@@ -306,9 +306,10 @@ void preprocess(const kf_intrinsics& k0, const kf_preprocess_params& p, const ui
           if (qx < 0 || qx >= W) continue;
           const R dq = d0[qy * W + qx];
           if (dq == R(0)) continue;
-          const R diff = dq - dc;
-          const R arg = (R)(dx * dx + dy * dy) * inv2ss + (diff * diff) * inv2sr;
-          const R w = exp_neg(arg);
+          // spatial factor times range factor, the range taken on the depth codes
+          const int du = std::abs((int)raw[qy * W + qx] - (int)raw[idx]);
+          const R rm = (R)du * scale;
+          const R w = exp_neg((R)(dx * dx + dy * dy) * inv2ss) * exp_neg((rm * rm) * inv2sr);
           sw = sw + w;
           swd = swd + w * dq;
         }
