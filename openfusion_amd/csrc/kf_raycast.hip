@@ -12,6 +12,8 @@
 // 8x8 tile (rays in a wave stay coherent in the volume).  Block ids are remapped
 // so each of the 8 XCDs (which receive blocks round-robin) owns a contiguous
 // band of image rows and its L2 sees neighbouring rays.
+#include <type_traits>
+
 #include "kf_device.hpp"
 #include "kf_internal.hpp"
 
@@ -34,124 +36,162 @@ struct RayArgs {
   float fx, fy, cx, cy;
   int W, H;
   float min_depth, step_far_m, step_near_m;
+  float inv_step_far;  // 1 / step_far_m (estimates only)
   int max_steps;
   int blocks_x, blocks_y;
 };
 
+// f of the voxel at byte address p (the TSDF half of the voxel record).
 template <int FMT>
-__device__ __forceinline__ float vox_f(const void* vox, size_t i) {
-  if (FMT == KF_VOXEL_F16)
-    return half_bits_to_float(static_cast<const uint32_t*>(vox)[i] & 0xffffu);
-  return static_cast<const float2*>(vox)[i].x;
+__device__ __forceinline__ float vox_at(const char* p) {
+  if (FMT == KF_VOXEL_F16) return half_bits_to_float(*reinterpret_cast<const uint32_t*>(p) & 0xffffu);
+  return *reinterpret_cast<const float*>(p);
 }
 
 __device__ __forceinline__ float lerp(float a, float b, float t) { return a + (b - a) * t; }
 
-// Trilinear sample of f at world point p; false outside the voxel-centre domain.
-// When the brick of the base voxel is clear in the occupancy map, all eight
-// corners hold f == 1 and the interpolation is exactly 1 (a + (b - a) t with
-// a = b = 1), so the voxel loads are skipped: the value is the same.
-template <int FMT>
-__device__ __forceinline__ bool sample(const RayArgs& a, f3 p, float& out) {
-  const float gx = (p.x - a.ox) * a.inv_h - 0.5f;
-  const float gy = (p.y - a.oy) * a.inv_h - 0.5f;
-  const float gz = (p.z - a.oz) * a.inv_h - 0.5f;
-  const float lim = (float)(a.n - 1);
-  if (!(gx >= 0.0f && gx < lim && gy >= 0.0f && gy < lim && gz >= 0.0f && gz < lim)) return false;
-  const int ix = (int)gx, iy = (int)gy, iz = (int)gz;
-  if (a.occ) {
-    const unsigned long long w = a.occ[(ix >> 3) * a.occ_nb + (iy >> 3)];
-    if (!((w >> (iz >> 3)) & 1ull)) {
-      out = 1.0f;
-      return true;
-    }
-  }
-  const float tx = gx - (float)ix, ty = gy - (float)iy, tz = gz - (float)iz;
-  const size_t n = (size_t)a.n;
-  const size_t b = ((size_t)ix * n + (size_t)iy) * n + (size_t)iz;
-  const size_t sy = n, sx = n * n;
-  const float c000 = vox_f<FMT>(a.vox, b), c001 = vox_f<FMT>(a.vox, b + 1);
-  const float c010 = vox_f<FMT>(a.vox, b + sy), c011 = vox_f<FMT>(a.vox, b + sy + 1);
-  const float c100 = vox_f<FMT>(a.vox, b + sx), c101 = vox_f<FMT>(a.vox, b + sx + 1);
-  const float c110 = vox_f<FMT>(a.vox, b + sx + sy), c111 = vox_f<FMT>(a.vox, b + sx + sy + 1);
-  const float c00 = lerp(c000, c001, tz), c01 = lerp(c010, c011, tz);
-  const float c10 = lerp(c100, c101, tz), c11 = lerp(c110, c111, tz);
-  const float c0 = lerp(c00, c01, ty), c1 = lerp(c10, c11, ty);
-  out = lerp(c0, c1, tx);
-  return true;
+// Byte offset of voxel (ix, iy, iz) in the x-major volume.  OFF32: the volume's
+// byte size fits in 32 bits, so the offset is one 32-bit register (24-bit
+// multiplies, exact for n <= 1024) and every load below is a uniform base address
+// plus that register.
+template <bool OFF32>
+using voff_t = typename std::conditional<OFF32, unsigned, size_t>::type;
+
+// x returned through an empty asm statement: the compiler cannot see its range, so
+// it neither widens the arithmetic that uses it to 64 bits nor fuses a 24-bit
+// multiply and add into a 64-bit multiply-add.
+__device__ __forceinline__ unsigned opaque_u32(unsigned x) {
+  asm("" : "+v"(x));
+  return x;
 }
 
-// K trilinear samples at o + t_j dir with the arithmetic of sample(), in three
-// phases so that a batch costs about one occupancy-map latency plus one voxel
-// latency instead of K of each: all map words are loaded, then the corners of
-// every sample whose brick is occupied, then the interpolations.  ok[j] is false
-// outside the voxel-centre domain.
-template <int FMT, int K>
+template <int FMT, bool OFF32>
+__device__ __forceinline__ voff_t<OFF32> vox_offset(int n, int ix, int iy, int iz) {
+  constexpr unsigned kB = FMT == KF_VOXEL_F16 ? 4u : 8u;
+  if constexpr (OFF32) {
+    const unsigned un = (unsigned)n;
+    const unsigned row = opaque_u32(__umul24((unsigned)ix, un)) + (unsigned)iy;
+    return (opaque_u32(__umul24(row, un)) + (unsigned)iz) * kB;
+  } else {
+    return ((((size_t)ix * (size_t)n) + (size_t)iy) * (size_t)n + (size_t)iz) * kB;
+  }
+}
+
+// The eight corners of the trilinear cell whose base voxel is at byte offset o:
+// c[0..7] = f at (x,y,z), (x,y,z+1), (x,y+1,z), (x,y+1,z+1), (x+1,y,z), ...
+template <int FMT, bool OFF32>
+__device__ __forceinline__ void load_corners(const RayArgs& a, voff_t<OFF32> o, float* c) {
+  constexpr unsigned kB = FMT == KF_VOXEL_F16 ? 4u : 8u;
+  // an opaque 32-bit register keeps the base-register + 32-bit-offset load form
+  if constexpr (OFF32) o = opaque_u32(o);
+  const char* b00 = static_cast<const char*>(a.vox);
+  const char* b01 = b00 + (size_t)a.n * kB;        // y + 1
+  const char* b10 = b00 + (size_t)a.n * a.n * kB;  // x + 1
+  const char* b11 = b10 + (size_t)a.n * kB;        // x + 1, y + 1
+  c[0] = vox_at<FMT>(b00 + o);
+  c[1] = vox_at<FMT>(b00 + o + kB);
+  c[2] = vox_at<FMT>(b01 + o);
+  c[3] = vox_at<FMT>(b01 + o + kB);
+  c[4] = vox_at<FMT>(b10 + o);
+  c[5] = vox_at<FMT>(b10 + o + kB);
+  c[6] = vox_at<FMT>(b11 + o);
+  c[7] = vox_at<FMT>(b11 + o + kB);
+}
+
+// K trilinear samples of f at world points pts[j]; ok[j] is false outside the
+// voxel-centre domain.  Grid coordinate g = (p - origin) / h - 1/2, base voxel
+// floor(g), weights g - floor(g).  When the brick of the base voxel is clear in
+// the occupancy map, all eight corners hold f == 1 and the interpolation is
+// exactly 1 (a + (b - a) t with a = b = 1), so the voxel loads are skipped: the
+// value is the same.  Three phases, so that a batch costs about one map latency
+// plus one voxel latency instead of K of each: the map words of all samples, then
+// the corners of all samples (one wave-uniform branch; lanes whose sample needs
+// no corners read voxel 0 instead, which stays cached), then the interpolations.
+// Returns the number of samples that needed corners.
+template <int FMT, bool OFF32, int K>
 __device__ __forceinline__ int sample_points(const RayArgs& a, const f3* pts, bool* ok, float* fk,
                                              bool* clear = nullptr) {
   const float lim = (float)(a.n - 1);
   float gx[K], gy[K], gz[K];
   int ix[K], iy[K], iz[K];
-  unsigned long long w[K];
+  bool need[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     const f3 p = pts[j];
     gx[j] = (p.x - a.ox) * a.inv_h - 0.5f;
     gy[j] = (p.y - a.oy) * a.inv_h - 0.5f;
     gz[j] = (p.z - a.oz) * a.inv_h - 0.5f;
-    ok[j] = gx[j] >= 0.0f && gx[j] < lim && gy[j] >= 0.0f && gy[j] < lim && gz[j] >= 0.0f && gz[j] < lim;
+    // positions are finite (finite pose, t inside the slab interval)
+    ok[j] = fminf(fminf(gx[j], gy[j]), gz[j]) >= 0.0f && fmaxf(fmaxf(gx[j], gy[j]), gz[j]) < lim;
     ix[j] = ok[j] ? (int)gx[j] : 0;
     iy[j] = ok[j] ? (int)gy[j] : 0;
     iz[j] = ok[j] ? (int)gz[j] : 0;
   }
+  if (a.occ) {
+    // the 32-bit half of the brick column word that holds bit iz / 8
+    const char* occ = reinterpret_cast<const char*>(a.occ);
+    unsigned w[K];
 #pragma unroll
-  for (int j = 0; j < K; ++j) w[j] = a.occ ? a.occ[(ix[j] >> 3) * a.occ_nb + (iy[j] >> 3)] : ~0ull;
-  float c[K][8];
-  bool need[K];
-  const size_t n = (size_t)a.n, sy = n, sx = n * n;
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    need[j] = ok[j] && ((w[j] >> (iz[j] >> 3)) & 1ull);
-    if (clear) clear[j] = ok[j] && !need[j] && a.occ;
-    if (need[j]) {
-      const size_t b = ((size_t)ix[j] * n + (size_t)iy[j]) * n + (size_t)iz[j];
-      c[j][0] = vox_f<FMT>(a.vox, b);
-      c[j][1] = vox_f<FMT>(a.vox, b + 1);
-      c[j][2] = vox_f<FMT>(a.vox, b + sy);
-      c[j][3] = vox_f<FMT>(a.vox, b + sy + 1);
-      c[j][4] = vox_f<FMT>(a.vox, b + sx);
-      c[j][5] = vox_f<FMT>(a.vox, b + sx + 1);
-      c[j][6] = vox_f<FMT>(a.vox, b + sx + sy);
-      c[j][7] = vox_f<FMT>(a.vox, b + sx + sy + 1);
+    for (int j = 0; j < K; ++j) {
+      const unsigned col = opaque_u32(__umul24((unsigned)(ix[j] >> 3), (unsigned)a.occ_nb)) + (unsigned)(iy[j] >> 3);
+      const unsigned off = opaque_u32(((col << 1) | (unsigned)(iz[j] >> 8)) * 4u);
+      w[j] = *reinterpret_cast<const unsigned*>(occ + off);
     }
+#pragma unroll
+    for (int j = 0; j < K; ++j) need[j] = ok[j] && ((w[j] >> ((iz[j] >> 3) & 31)) & 1u);
+  } else {
+#pragma unroll
+    for (int j = 0; j < K; ++j) need[j] = ok[j];
   }
-  int loads = 0;
+  int loads = 0, any = 0;
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     fk[j] = 1.0f;
+    if (clear) clear[j] = ok[j] && !need[j];
     loads += need[j] ? 1 : 0;
-    if (need[j]) {
-      const float tx = gx[j] - (float)ix[j], ty = gy[j] - (float)iy[j], tz = gz[j] - (float)iz[j];
+    any |= need[j] ? 1 : 0;
+  }
+  if (__any(any)) {
+    float c[K][8];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const voff_t<OFF32> o = vox_offset<FMT, OFF32>(a.n, ix[j], iy[j], iz[j]);
+      load_corners<FMT, OFF32>(a, need[j] ? o : (voff_t<OFF32>)0, c[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      // g >= 0 where needed, so fract(g) = g - floor(g) = g - (float)(int)g exactly
+      const float tx = __builtin_amdgcn_fractf(gx[j]), ty = __builtin_amdgcn_fractf(gy[j]),
+                  tz = __builtin_amdgcn_fractf(gz[j]);
       const float c00 = lerp(c[j][0], c[j][1], tz), c01 = lerp(c[j][2], c[j][3], tz);
       const float c10 = lerp(c[j][4], c[j][5], tz), c11 = lerp(c[j][6], c[j][7], tz);
       const float c0 = lerp(c00, c01, ty), c1 = lerp(c10, c11, ty);
-      fk[j] = lerp(c0, c1, tx);
+      const float v = lerp(c0, c1, tx);
+      fk[j] = need[j] ? v : 1.0f;
     }
   }
   return loads;
 }
 
+// One trilinear sample at world point p (sample_points with K = 1).
+template <int FMT, bool OFF32>
+__device__ __forceinline__ bool sample(const RayArgs& a, f3 p, float& out) {
+  bool ok;
+  sample_points<FMT, OFF32, 1>(a, &p, &ok, &out);
+  return ok;
+}
+
 // K samples along the ray at parameters tk (positions o + t dir, as in sample()).
-template <int FMT, int K>
+template <int FMT, bool OFF32, int K>
 __device__ __forceinline__ int sample_batch(const RayArgs& a, f3 o, f3 dir, const float* tk, bool* ok,
                                             float* fk, bool* clear) {
   f3 pts[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) pts[j] = f3{o.x + tk[j] * dir.x, o.y + tk[j] * dir.y, o.z + tk[j] * dir.z};
-  return sample_points<FMT, K>(a, pts, ok, fk, clear);
+  return sample_points<FMT, OFF32, K>(a, pts, ok, fk, clear);
 }
 
-template <int FMT>
+template <int FMT, bool OFF32>
 __global__ __launch_bounds__(256) void raycast(RayArgs a) {
   // XCD-aware block remap: logical = (xcd band) + (position within band).
   const int nb = a.blocks_x * a.blocks_y;
@@ -211,8 +251,8 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
                      f3{ph.x, ph.y - a.h, ph.z}, f3{ph.x, ph.y, ph.z + a.h}, f3{ph.x, ph.y, ph.z - a.h}};
     bool okq[6];
     float fq[6];
-    sample_points<FMT, 3>(a, q, okq, fq);
-    sample_points<FMT, 3>(a, q + 3, okq + 3, fq + 3);
+    sample_points<FMT, OFF32, 3>(a, q, okq, fq);
+    sample_points<FMT, OFF32, 3>(a, q + 3, okq + 3, fq + 3);
     const bool g = okq[0] && okq[1] && okq[2] && okq[3] && okq[4] && okq[5];
     const float gxp = fq[0], gxm = fq[1], gyp = fq[2], gym = fq[3], gzp = fq[4], gzm = fq[5];
     if (g) {
@@ -237,15 +277,8 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
     // measured slower.
     constexpr int kSpec = 2;
     float t = tnear, fp;
-    bool ok = sample<FMT>(a, f3{o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z}, fp) && fp >= 0.0f;
+    bool ok = sample<FMT, OFF32>(a, f3{o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z}, fp) && fp >= 0.0f;
     int s = 0;
-    // Free-space skip.  From a far-mode position t (sample 1) whose base voxel lies
-    // in a clear super-brick, advance k far steps at once: the k positions are the
-    // sequential ones (t += step, k additions), and the grid coordinate of a
-    // position is a monotone function of t on each axis, so if the last one is
-    // still in the super-brick (checked with the exact sample() arithmetic) every
-    // intermediate one is too, and each of their samples is exactly 1.  The step
-    // count, the position and the value are those of the step-by-step march.
     // Free-space skip.  From a far-mode position t (sample 1) whose base voxel lies
     // in a clear box — its 32^3 super-brick when that is clear, else its brick when
     // the brick is clear — advance k far steps at once: the k positions are the
@@ -265,18 +298,20 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
       const float lx = (float)((ix >> sh) << sh), hx = fminf((float)(((ix >> sh) + 1) << sh), lim);
       const float ly = (float)((iy >> sh) << sh), hy = fminf((float)(((iy >> sh) + 1) << sh), lim);
       const float lz = (float)((iz >> sh) << sh), hz = fminf((float)(((iz >> sh) + 1) << sh), lim);
-      // approximate exit parameter of the box (grid value G at world (G + 0.5) h + origin)
-      float te = tfar;
-      if (dir.x > 0.0f) te = fminf(te, ((hx + 0.5f) * a.h + a.ox - o.x) / dir.x);
-      if (dir.x < 0.0f) te = fminf(te, ((lx + 0.5f) * a.h + a.ox - o.x) / dir.x);
-      if (dir.y > 0.0f) te = fminf(te, ((hy + 0.5f) * a.h + a.oy - o.y) / dir.y);
-      if (dir.y < 0.0f) te = fminf(te, ((ly + 0.5f) * a.h + a.oy - o.y) / dir.y);
-      if (dir.z > 0.0f) te = fminf(te, ((hz + 0.5f) * a.h + a.oz - o.z) / dir.z);
-      if (dir.z < 0.0f) te = fminf(te, ((lz + 0.5f) * a.h + a.oz - o.z) / dir.z);
-      int k = (int)((te - t_) / a.step_far_m) - 1;
+      // Approximate exit parameter of the box (grid value G sits at world
+      // (G + 0.5) h + origin), with reciprocals: only the estimate of k depends on
+      // it, and the skip is verified with the exact arithmetic below.
+      auto exit_t = [&](float d, float lo_, float hi_, float org, float oc) {
+        const float plane = ((d > 0.0f ? hi_ : lo_) + 0.5f) * a.h + org - oc;
+        return d != 0.0f ? plane * __builtin_amdgcn_rcpf(d) : 3.0e38f;
+      };
+      const float te = fminf(fminf(tfar, exit_t(dir.x, lx, hx, a.ox, o.x)),
+                             fminf(exit_t(dir.y, ly, hy, a.oy, o.y), exit_t(dir.z, lz, hz, a.oz, o.z)));
+      int k = (int)fminf((te - t_) * a.inv_step_far, (float)a.max_steps) - 1;
       k = min(k, a.max_steps - s_);
       if (k <= 0) return;
       float tt = t_;
+#pragma unroll 4
       for (int j = 0; j < k; ++j) tt = tt + a.step_far_m;
       const float ex = ((o.x + tt * dir.x) - a.ox) * a.inv_h - 0.5f;
       const float ey = ((o.y + tt * dir.y) - a.oy) * a.inv_h - 0.5f;
@@ -301,7 +336,7 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
         tt = tt + st;
         tk[j] = tt;
       }
-      n_loads += sample_batch<FMT, kSpec>(a, o, dir, tk, vk, fk, ck);
+      n_loads += sample_batch<FMT, OFF32, kSpec>(a, o, dir, tk, vk, fk, ck);
       ++n_batches;
 #pragma unroll
       for (int j = 0; j < kSpec; ++j) vk[j] = vk[j] && tk[j] <= tfar;
@@ -434,15 +469,25 @@ kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_r
   a.min_depth = p.min_depth_m;
   a.step_far_m = p.step_far * d.trunc_m;
   a.step_near_m = p.step_near * a.h;
+  a.inv_step_far = 1.0f / a.step_far_m;
   a.max_steps = p.max_steps;
   a.blocks_x = div_up(k0.width, 16);
   a.blocks_y = div_up(k0.height, 16);
   const int nb = a.blocks_x * a.blocks_y;
   const int grid = ((nb + kXcds - 1) / kXcds) * kXcds;
-  if (d.format == KF_VOXEL_F16)
-    hipLaunchKernelGGL(raycast<KF_VOXEL_F16>, dim3(grid), dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL(raycast<KF_VOXEL_F32>, dim3(grid), dim3(256), 0, s, a);
+  // 32-bit voxel offsets when every byte offset of the volume fits in 32 bits
+  const bool off32 = v->bytes <= 0xffffffffull && d.n <= 1024;
+  if (d.format == KF_VOXEL_F16) {
+    if (off32)
+      hipLaunchKernelGGL((raycast<KF_VOXEL_F16, true>), dim3(grid), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((raycast<KF_VOXEL_F16, false>), dim3(grid), dim3(256), 0, s, a);
+  } else {
+    if (off32)
+      hipLaunchKernelGGL((raycast<KF_VOXEL_F32, true>), dim3(grid), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((raycast<KF_VOXEL_F32, false>), dim3(grid), dim3(256), 0, s, a);
+  }
   KF_TRY(check_launch("raycast"));
   if (after_raycast) KF_HIP(hipEventRecord(after_raycast, s));
   if (model.levels > 1) {
