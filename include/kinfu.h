@@ -263,6 +263,9 @@ typedef struct kf_icp_params {
   float dist_thresh_m;
   float normal_thresh_deg;
   int32_t compensated;                /* 1: Kahan FP32 partials (default); 0: plain FP32 partials */
+  int32_t single_launch;              /* 1: the whole call is one launch, iterations separated by a
+                                         grid barrier (results identical); 0: one launch per
+                                         iteration (default) */
 } kf_icp_params;
 
 /* Per-call tracking statistics (device struct; read back with kf_memcpy_dtoh). */

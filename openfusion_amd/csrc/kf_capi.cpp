@@ -58,6 +58,7 @@ static kf_status check_icp_params(const kf_icp_params* p) {
     KF_CHECK_ARG(p->iterations[l] >= 0 && p->iterations[l] <= 100, "iterations must be 0..100");
   KF_CHECK_ARG(p->dist_thresh_m > 0.0f, "dist_thresh must be positive");
   KF_CHECK_ARG(p->normal_thresh_deg > 0.0f && p->normal_thresh_deg < 90.0f, "normal_thresh must be in (0,90)");
+  KF_CHECK_ARG(p->single_launch == 0 || p->single_launch == 1, "single_launch must be 0 or 1");
   return KF_OK;
 }
 

@@ -37,8 +37,11 @@ constexpr int kThreads = 256;
 constexpr double kPivotRelEps = 1e-9;
 constexpr int kMinInliers = 6;
 constexpr int kFinUnroll = 8;       // partial-slot loads in flight per thread in the final sum
-constexpr int kSolveThreads = 1024;  // one block: 64 groups x 16 float4 columns of the slots
 constexpr int kStepThreads = 512;    // tracking step: 8 waves per block
+// One-block sums and solves use the step's block size, so that every slot sum (in
+// a step, the final one-block launch or the single-launch kernel) adds in the same
+// order: 32 groups x 16 float4 columns of the slots.
+constexpr int kSolveThreads = kStepThreads;
 
 struct IcpArgs {
   const float *lvx, *lvy, *lvz, *lnx, *lny, *lnz;  // live level maps (camera frame)
@@ -593,6 +596,164 @@ __global__ __launch_bounds__(kSolveThreads) void icp_finish(IcpArgs a, StepArgs 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Single-launch tracking: the iterations of the icp_step chain and icp_finish in
+// one launch, for calls that are usually skipped (the prior's fallback), where a
+// skipped call then costs one launch instead of one per iteration.  Block b
+// computes the slots b, b + G, ... of each iteration's step grid (same grid size,
+// same block_partial arguments, hence the same slots), every block sums the
+// previous iteration's slots and solves as icp_step does, and a grid barrier
+// replaces the kernel boundary between iterations, so poses and statistics are
+// those of the multi-launch chain bit for bit.  Every block keeps the pose in LDS;
+// only the last one is written.
+//
+// The barrier needs all G blocks resident at once: the launcher caps G at the
+// device's co-resident block count for this kernel.  Its wait is bounded: a block
+// that waits longer than kBarrierPolls polls gives up, the control block is marked
+// broken and that call and every later single-launch call of the tracker report
+// status 2 (the waits of the other blocks end the same way), so the grid always
+// drains.
+constexpr unsigned kBarrierPolls = 1u << 20;
+
+struct ChainArgs {
+  IcpArgs lv[KF_MAX_LEVELS];
+  int levels;
+  int iters[KF_MAX_LEVELS];
+  int blocks[KF_MAX_LEVELS];  // step grid (slot count) per level, as in the multi-launch chain
+  const kf_pose* guess;
+  kf_pose* out;
+  float* slots[2];
+  kf_icp_stats* user;  // skip flag; receives a copy of the statistics
+  kf_icp_stats* st;
+  kf_icp_ctrl* ctrl;
+};
+
+// Grid barrier: the last block to arrive resets the arrival count and advances the
+// generation word that the others poll.  Device-scope fences order each block's
+// slot stores before its arrival and the other blocks' slots after the wait.
+// Returns false (for the whole block) when the wait timed out.
+__device__ __forceinline__ bool grid_barrier(kf_icp_ctrl* ctrl, unsigned nblocks) {
+  __shared__ int s_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    // read before arriving: the generation cannot advance until this block arrives
+    const unsigned g = __hip_atomic_load(&ctrl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const unsigned prev = atomicAdd(&ctrl->arrive, 1u);
+    if (prev == nblocks - 1) {
+      __hip_atomic_store(&ctrl->arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __threadfence();
+      atomicAdd(&ctrl->gen, 1u);
+    } else {
+      unsigned polls = 0;
+      while (__hip_atomic_load(&ctrl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        if (++polls > kBarrierPolls) {
+          ok = 0;
+          __hip_atomic_store(&ctrl->broken, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __threadfence();
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(kStepThreads) void icp_chain(ChainArgs C) {
+  __shared__ double sum[kTerms + 2], T[2][16];
+  __shared__ kf_posef s_wc, s_cw;
+  const int tid = threadIdx.x;
+  const bool lead = blockIdx.x == 0;
+  if (C.user && C.user->skip) return;  // uniform
+  kf_icp_stats* st = C.st;
+  if (C.ctrl->broken) {  // uniform: written by an earlier launch
+    if (lead && tid == 0) {
+      kf_icp_stats z = {};
+      z.status = 2;
+      *st = z;
+      if (C.user && C.user != st) *C.user = z;
+    }
+    if (lead) copy_pose(C.guess, C.out);
+    return;
+  }
+  if (tid < 16) T[0][tid] = C.guess->T_wc[tid];
+  if (tid < 12) {
+    if (tid < 9) s_wc.r[tid] = C.guess->wc.r[tid];
+    else s_wc.t[tid - 9] = C.guess->wc.t[tid - 9];
+  } else if (tid < 24) {
+    if (tid < 21) s_cw.r[tid - 12] = C.guess->cw.r[tid - 12];
+    else s_cw.t[tid - 21] = C.guess->cw.t[tid - 21];
+  }
+  if (lead && tid == 32) {
+    kf_icp_stats z = {};
+    *st = z;
+  }
+  __syncthreads();
+  int cur = 0, executed = 0, nb_prev = 0, lvl_prev = 0, par = 0, steps = 0;
+  bool stop = false, timeout = false;
+  for (int l = C.levels - 1; l >= 0 && !stop; --l) {
+    const IcpArgs& a = C.lv[l];
+    const int nb = C.blocks[l];
+    for (int it = 0; it < C.iters[l]; ++it) {
+      if (steps > 0) {  // solve the previous iteration's slots (as icp_step)
+        sum_slots<kStepThreads>(C.slots[par ^ 1], nb_prev, sum);
+        if (!solve_compose(sum, T[cur], lead ? st : nullptr, lvl_prev, T[cur ^ 1], &s_wc, &s_cw)) {
+          stop = true;  // uniform: every block solved the same system
+          break;
+        }
+        cur ^= 1;
+        ++executed;
+      }
+      __syncthreads();
+      for (int vb = blockIdx.x; vb < nb; vb += gridDim.x) {
+        block_partial<COMP, kStepThreads / kWave>(a, s_wc, s_cw, vb, nb, C.slots[par] + (size_t)vb * kSlot);
+        __syncthreads();
+      }
+      if (!grid_barrier(C.ctrl, gridDim.x)) {
+        stop = timeout = true;
+        break;
+      }
+      nb_prev = nb;
+      lvl_prev = l;
+      par ^= 1;
+      ++steps;
+    }
+  }
+  if (!stop && steps > 0) {  // the last iteration's update (as icp_finish)
+    sum_slots<kStepThreads>(C.slots[par ^ 1], nb_prev, sum);
+    if (solve_compose(sum, T[cur], lead ? st : nullptr, lvl_prev, T[cur ^ 1], &s_wc, &s_cw)) {
+      cur ^= 1;
+      ++executed;
+    }
+  }
+  if (!lead) return;
+  __syncthreads();
+  kf_pose* P = C.out;
+  if (tid < 16) P->T_wc[tid] = T[cur][tid];
+  else if (tid >= 64 && tid < 76) {
+    const int q = tid - 64;
+    if (q < 9) P->wc.r[q] = s_wc.r[q]; else P->wc.t[q - 9] = s_wc.t[q - 9];
+  } else if (tid >= 128 && tid < 140) {
+    const int q = tid - 128;
+    if (q < 9) P->cw.r[q] = s_cw.r[q]; else P->cw.t[q - 9] = s_cw.t[q - 9];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    st->iterations = executed;
+    if (timeout) st->status = 2;
+    if (C.user && C.user != st) {
+      kf_icp_stats c = *st;
+      c.skip = 0;
+      *C.user = c;
+    }
+  }
+}
+
 __global__ void pose_const_velocity(const kf_pose* p2, const kf_pose* p1, kf_pose* g) {
   if (threadIdx.x != 0) return;
   double Ti[16], Dl[16], G[16];
@@ -663,12 +824,47 @@ kf_status check_maps(const kf_tracker* t, const kf_frame_maps& live, const kf_fr
 
 kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_frame_maps& model,
                            const kf_pose* d_guess, kf_pose* d_out, kf_icp_stats* d_stats,
-                           hipStream_t s) {
+                           hipStream_t s, bool single_launch) {
   KF_CHECK_ARG(d_guess && d_out, "guess/out pose is NULL");
   const kf_icp_params& p = t->params;
   KF_TRY(check_maps(t, live, model, p.levels));
   kf_icp_stats* st = t->d_stats_tmp;
   float* slots[2] = {t->d_partials, t->d_partials + (size_t)t->max_blocks * kSlot};
+  if (single_launch || p.single_launch) {
+    ChainArgs C = {};
+    C.levels = p.levels;
+    int grid = 1;
+    for (int l = 0; l < p.levels; ++l) {
+      C.lv[l] = level_args(t, l, live, model);
+      C.iters[l] = p.iterations[l];
+      C.blocks[l] = step_blocks(t, C.lv[l].W, C.lv[l].H, l);
+      if (C.iters[l] > 0 && C.blocks[l] > grid) grid = C.blocks[l];
+    }
+    if (t->chain_capacity < 0) {  // blocks of this kernel that can be resident at once
+      int dev = 0, per_cu = 0, cus = 0;
+      KF_HIP(hipGetDevice(&dev));
+      KF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      if (p.compensated)
+        KF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, icp_chain<true>, kStepThreads, 0));
+      else
+        KF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, icp_chain<false>, kStepThreads, 0));
+      t->chain_capacity = per_cu * cus;
+    }
+    KF_CHECK_ARG(t->chain_capacity >= 1, "single-launch ICP cannot be resident on this device");
+    if (grid > t->chain_capacity) grid = t->chain_capacity;
+    C.guess = d_guess;
+    C.out = d_out;
+    C.slots[0] = slots[0];
+    C.slots[1] = slots[1];
+    C.user = d_stats;
+    C.st = st;
+    C.ctrl = t->d_ctrl;
+    if (p.compensated)
+      hipLaunchKernelGGL(icp_chain<true>, dim3(grid), dim3(kStepThreads), 0, s, C);
+    else
+      hipLaunchKernelGGL(icp_chain<false>, dim3(grid), dim3(kStepThreads), 0, s, C);
+    return check_launch("icp_chain");
+  }
   StepArgs S = {};
   S.user = d_stats;
   S.st = st;

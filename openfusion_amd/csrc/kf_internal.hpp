@@ -61,10 +61,12 @@ struct kf_volume {
 
 // ICP scratch and per-iteration control block (device memory).
 struct kf_icp_ctrl {
-  unsigned int reserved;
+  unsigned int arrive;  // single-launch grid barrier: blocks arrived at the current barrier
   int stop;             // set when a system was singular: later iterations skip
   int executed;         // iterations that produced an update
-  int pad;
+  unsigned int gen;     // single-launch grid barrier: generation (advanced by the last arrival)
+  int broken;           // a barrier wait timed out: single-launch calls report status 2
+  int pad[3];
 };
 
 struct kf_tracker {
@@ -75,6 +77,7 @@ struct kf_tracker {
   kf_icp_ctrl* d_ctrl = nullptr;
   kf_pose* d_work = nullptr;     // two poses: the step outputs alternate between them
   kf_icp_stats* d_stats_tmp = nullptr;  // used when the caller passes none
+  int chain_capacity = -1;      // co-resident blocks of the single-launch kernel (-1: not queried)
   double* d_sums_tmp = nullptr;
 };
 
@@ -96,9 +99,11 @@ kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_r
                          const kf_pose* d_pose, kf_frame_maps& model, unsigned long long* d_steps,
                          hipStream_t s, hipEvent_t after_raycast = nullptr,
                          unsigned long long* d_diag = nullptr);
+// single_launch: the whole call as one launch (also when the tracker's parameters
+// do not ask for it); used where most calls are skipped by the statistics' flag.
 kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_frame_maps& model,
                            const kf_pose* d_guess, kf_pose* d_out, kf_icp_stats* d_stats,
-                           hipStream_t s);
+                           hipStream_t s, bool single_launch = false);
 kf_status launch_icp_reduce(kf_tracker* t, int level, const kf_frame_maps& live,
                             const kf_frame_maps& model, const kf_pose* d_est, double* d_sums,
                             uint8_t* d_mask, hipStream_t s);

@@ -229,8 +229,10 @@ kf_status record_track_prior(kf_pipeline* p, hipStream_t s) {
   hipLaunchKernelGGL(prior_decide, dim3(1), dim3(64), 0, s, p->d_prior, (const kf_icp_stats*)&p->d_stats[1],
                      &p->d_stats[0], p->d_poses, d.fallback_residual_ratio, d.fallback_min_inlier_frac);
   KF_TRY(check_launch("prior_decide"));
+  // the fallback: plain ICP from the constant-velocity guess, skipped (by the flag
+  // prior_decide sets) in most frames, hence one launch for the whole call
   KF_TRY(launch_icp_track(p->tracker, p->live, p->model, &p->d_poses[kGuess], &p->d_poses[kCur], &p->d_stats[0],
-                          s));
+                          s, true));
   hipLaunchKernelGGL(prior_post, dim3(1), dim3(64), 0, s, p->d_prior, (const kf_icp_stats*)&p->d_stats[1],
                      (const kf_icp_stats*)&p->d_stats[0], (const kf_pose*)p->d_poses);
   return check_launch("prior_post");

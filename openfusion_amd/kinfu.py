@@ -80,7 +80,7 @@ class RaycastParams(C.Structure):
 class IcpParams(C.Structure):
     _fields_ = [("levels", C.c_int32), ("iterations", C.c_int32 * MAX_LEVELS),
                 ("dist_thresh_m", C.c_float), ("normal_thresh_deg", C.c_float),
-                ("compensated", C.c_int32)]
+                ("compensated", C.c_int32), ("single_launch", C.c_int32)]
 
 
 class IcpStats(C.Structure):

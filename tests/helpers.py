@@ -28,12 +28,13 @@ def ray_params(dmin=0.3, far=0.8, near=0.5, max_steps=2048) -> kf.RaycastParams:
     return kf.RaycastParams(dmin, far, near, max_steps)
 
 
-def icp_params(levels=3, iters=(10, 5, 4), dist=0.1, deg=20.0, comp=1) -> kf.IcpParams:
+def icp_params(levels=3, iters=(10, 5, 4), dist=0.1, deg=20.0, comp=1, single=0) -> kf.IcpParams:
     p = kf.IcpParams()
     p.levels = levels
     for i in range(3):
         p.iterations[i] = iters[i] if i < len(iters) else 0
     p.dist_thresh_m, p.normal_thresh_deg, p.compensated = dist, deg, comp
+    p.single_launch = single
     return p
 
 

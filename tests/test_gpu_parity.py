@@ -139,6 +139,49 @@ def test_icp_reduce_and_track_parity(gpu, kf, orc, synth, small):
         assert list(raw.wc.r) == list(ref.wc.r) and list(raw.cw.t) == list(ref.cw.t)
 
 
+def test_icp_single_launch_matches_multi_launch(gpu, kf, orc, synth, small):
+    """The single-launch tracking call (grid barrier between iterations) gives the
+    pose and statistics of the one-launch-per-iteration chain bit for bit: full
+    schedule, a level with no iterations, no iterations at all, and a singular first
+    system (status 1); a second call on the same tracker reuses the barrier words;
+    the skip flag leaves pose and statistics untouched."""
+    k = synth.intrinsics()
+    pr = synth.PRESETS["small"]
+    vol = kf.Volume(pr.n, pr.extent, pr.trunc, pr.max_weight, 0)
+    integ = kf.VolumeIntegrator(k, integ_params())
+    for f in range(3):
+        integ.Integrate(vol, _gpu_preprocess(kf, k, small.depth[f], 1), kf.DevicePose(small.poses[f]))
+    model = kf.DeviceMaps(k, 3, False)
+    kf.Raycaster(k, ray_params()).Raycast(vol, kf.DevicePose(small.poses[2]), model)
+    live = _gpu_preprocess(kf, k, small.depth[4])
+    guess = orc.constant_velocity(small.poses[1], small.poses[2])
+    cases = [dict(iters=(10, 5, 4)), dict(iters=(3, 0, 2)), dict(iters=(0, 0, 0)), dict(iters=(4, 3, 3), dist=1e-6)]
+    for comp in (1, 0):
+        for c in cases:
+            _say(f"icp single launch compensated={comp} case={c}")
+            res = []
+            for single in (0, 1):
+                icp = kf.ICP(k, icp_params(comp=comp, single=single, **c))
+                for _ in range(2 if single else 1):
+                    out = kf.DevicePose(guess)
+                    st = icp.Iterate_ICP(live, model, out)
+                    res.append((bytes(out.raw()), bytes(st), st.status, st.iterations))
+            assert res[1] == res[0] and res[2] == res[0], (res[0][2:], res[1][2:], res[2][2:])
+            if c.get("dist") == 1e-6:
+                assert res[0][2] == 1 and res[0][3] == 0
+            elif sum(c["iters"]) > 0:
+                assert res[0][2] == 0 and res[0][3] == sum(c["iters"])
+    # skip flag: the call does nothing
+    icp = kf.ICP(k, icp_params(single=1))
+    flag = kf.IcpStats()
+    flag.skip = 1
+    icp.stats.upload(np.frombuffer(bytes(flag), np.uint8))
+    out = kf.DevicePose(guess)
+    before = bytes(out.raw())
+    st = icp.Iterate_ICP(live, model, out)
+    assert bytes(out.raw()) == before and bytes(st) == bytes(flag)
+
+
 def _run_gpu_pipeline(kf, synth, stream, preset, use_graph=True, frames=None):
     d = synth.pipeline_desc(preset)
     d.use_graph = int(use_graph)
