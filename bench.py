@@ -341,7 +341,7 @@ def main() -> int:
         "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in res.items()},
         "observed_voxels_after_stage_run": sr["observed_voxels"],
-        "stages_ms": dict(zip(["preprocess", "guess+icp", "column_spans", "sweep", "raycast", "model_pyramid"],
+        "stages_ms": dict(zip(["preprocess", "guess+icp", "brick_classes", "sweep", "raycast", "model_pyramid"],
                               [round(float(x), 4) for x in stage])),
     }
     if "plain" in res:

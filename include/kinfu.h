@@ -404,7 +404,7 @@ kf_status kf_pipeline_get_maps(kf_pipeline* p, kf_frame_maps* live, kf_frame_map
 kf_status kf_pipeline_get_stream(kf_pipeline* p, kf_stream* s);
 /* Stage-timing mode (eager launches, hipEvents on the pipeline stream, voxel and
  * march-step counters).  Times (ms) of the last processed frame:
- * [preprocess, guess+icp, column spans, sweep kernel, raycast kernel,
+ * [preprocess, guess+icp, raylen tiles + brick classes, sweep kernel, raycast kernel,
  *  model pyramid + pose log]. */
 #define KF_STAGES 6
 kf_status kf_pipeline_set_stage_timing(kf_pipeline* p, int32_t on);

@@ -35,7 +35,7 @@ void pose_finish(kf_pose* p) { pose_finish_fields(p); }
 
 static kf_status check_intrinsics(const kf_intrinsics* k) {
   KF_CHECK_ARG(k != nullptr, "intrinsics is NULL");
-  KF_CHECK_ARG(k->width >= 16 && k->height >= 16 && k->width <= 8192 && k->height <= 8192,
+  KF_CHECK_ARG(k->width >= 16 && k->height >= 16 && k->width <= kf::kMaxImage && k->height <= kf::kMaxImage,
                "image size must be 16..8192");
   KF_CHECK_ARG(k->fx > 0.0f && k->fy > 0.0f, "focal lengths must be positive");
   return KF_OK;
@@ -263,11 +263,14 @@ extern "C" kf_status kf_volume_create(const kf_volume_desc* d, kf_volume** out) 
     delete v;
     return hip_fail(e, "hipMalloc(volume)");
   }
-  e = hipMalloc(&v->d_spans, n * n * sizeof(uint32_t));
+  e = hipMalloc(&v->d_bricks, (1 + (n / 8) * (n / 8) * (n / 8)) * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&v->d_rtiles, kf::kRayTiles * sizeof(float2));
   if (e != hipSuccess) {
     (void)hipFree(v->d_voxels);
+    (void)hipFree(v->d_bricks);
+    (void)hipFree(v->d_rtiles);
     delete v;
-    return hip_fail(e, "hipMalloc(spans)");
+    return hip_fail(e, "hipMalloc(sweep classes)");
   }
   if (d->n / 8 <= kf::kOccMaxBricks) {
     v->occ_nb = d->n / 8;
@@ -279,7 +282,8 @@ extern "C" kf_status kf_volume_create(const kf_volume_desc* d, kf_volume** out) 
     if (e == hipSuccess) v->d_occ2 = reinterpret_cast<unsigned*>(v->d_occ + (size_t)v->occ_nb * v->occ_nb);
     if (e != hipSuccess) {
       (void)hipFree(v->d_voxels);
-      (void)hipFree(v->d_spans);
+      (void)hipFree(v->d_bricks);
+      (void)hipFree(v->d_rtiles);
       delete v;
       return hip_fail(e, "hipMalloc(occupancy)");
     }
@@ -291,7 +295,8 @@ extern "C" kf_status kf_volume_create(const kf_volume_desc* d, kf_volume** out) 
   }
   if (st != KF_OK) {
     (void)hipFree(v->d_voxels);
-    (void)hipFree(v->d_spans);
+    (void)hipFree(v->d_bricks);
+    (void)hipFree(v->d_rtiles);
     (void)hipFree(v->d_occ);
     delete v;
     return st;
@@ -303,7 +308,8 @@ extern "C" kf_status kf_volume_create(const kf_volume_desc* d, kf_volume** out) 
 extern "C" kf_status kf_volume_destroy(kf_volume* v) {
   if (!v) return KF_OK;
   (void)hipFree(v->d_voxels);
-  (void)hipFree(v->d_spans);
+  (void)hipFree(v->d_bricks);
+  (void)hipFree(v->d_rtiles);
   (void)hipFree(v->d_occ);
   delete v;
   return KF_OK;

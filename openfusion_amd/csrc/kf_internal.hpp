@@ -46,7 +46,12 @@ struct kf_volume {
   kf_volume_desc desc;
   void* d_voxels = nullptr;  // n^3 * voxel_bytes
   size_t bytes = 0;
-  uint32_t* d_spans = nullptr;  // per (x,y) column: z0 | z1 << 16, conservative frustum span
+  // Sweep work list, rewritten by every sweep: the count, then one word per 8^3
+  // brick that the sweep must visit, bx << 24 | by << 16 | bz << 8 | class (free or
+  // exact); room for all nb^3 bricks.  And the raylen extrema of the 8x8 and 32x32
+  // pixel tiles the classes come from (sized for the largest image).
+  uint32_t* d_bricks = nullptr;
+  float2* d_rtiles = nullptr;
   // Brick occupancy map (volumes with n <= 512): bricks of 8^3 voxels, one 64-bit
   // word per (bx, by) brick column with bit bz.  A clear bit guarantees f == 1 for
   // every voxel of the brick extended by one voxel along +x, +y, +z (all corners
@@ -95,6 +100,10 @@ kf_status launch_volume_stats(const kf_volume* v, double* d_out, hipStream_t s);
 // Recomputes the occupancy map from the voxels when it is stale (no-op otherwise).
 kf_status ensure_occupancy(const kf_volume* v, hipStream_t s);
 constexpr int kOccMaxBricks = 64;  // bricks per axis held in one 64-bit word
+constexpr int kMaxImage = 8192;    // largest image side accepted by the API
+// raylen tiles: 8x8 then 32x32 pixels, (min, max) per tile
+constexpr size_t kRayTiles8 = (size_t)(kMaxImage / 8) * (kMaxImage / 8);
+constexpr size_t kRayTiles = kRayTiles8 + (size_t)(kMaxImage / 32) * (kMaxImage / 32);
 kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_raycast_params& p,
                          const kf_pose* d_pose, kf_frame_maps& model, unsigned long long* d_steps,
                          hipStream_t s, hipEvent_t after_raycast = nullptr,

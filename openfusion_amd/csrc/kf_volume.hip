@@ -6,15 +6,23 @@
 // Layout: voxel (x,y,z) at index (x*n + y)*n + z (z fastest), one 8-byte {f, w}
 // (FP32) or one 4-byte {half f, u16 w} (FP16) word per voxel, f = tsdf/T.
 //
-// The sweep is HBM-bound: it reads and writes only voxels that are updated.
-//   1. column_spans: one thread per (x,y) column computes, in FP64, a conservative
-//      z-interval outside which no voxel can pass the exact per-voxel test (the
-//      column is a line in camera space, the frustum is convex).
-//   2. integrate: one wavefront per column (waves stride over the columns) walks
-//      its span 64 voxels at a time (one voxel per lane, coalesced), evaluates the
-//      exact FP32 test per voxel (same expressions as the reference's per-voxel
-//      body, in metres and with a z-forward camera), loads a voxel only if it is
-//      updated and stores it only if its bits change.
+// The sweep reads and writes only voxels that are updated, and runs the exact
+// per-voxel test only where its outcome is not known for a whole brick:
+//   1. raylen_tiles: min / max raylen of every 8x8 and 32x32 pixel tile.
+//   2. brick_classify: one thread per 8^3 brick projects the brick's corner voxel
+//      centres in FP64 and, from the tile extrema under its footprint, marks it
+//      skip (no voxel can be updated), free (every voxel is updated with f = 1) or
+//      exact, with margins far above the FP32 error of the per-voxel test.
+//      Free and exact bricks are appended to a work list.
+//   3. integrate: a wavefront takes one listed brick at a time (waves stride over
+//      the list), a lane per (y, z) of the brick and the eight x-slices together:
+//      free bricks update every voxel with f = 1, exact bricks evaluate the FP32
+//      test of the reference's per-voxel body (in metres, z-forward camera) per
+//      voxel.  A voxel is stored only if its bits change.  Skip, free and exact give
+//      each voxel the outcome of the per-voxel test, so voxels, update masks and
+//      counts are those of testing every voxel.
+#include <type_traits>
+
 #include "kf_device.hpp"
 #include "kf_internal.hpp"
 
@@ -23,10 +31,17 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 constexpr int kSweepMaxBlocks = 4096;  // 16384 waves: two full waves of residency on 256 CUs
+constexpr int kSweepListBlocks = 2048;  // the sweep's waves (8192) stride over the brick list
+constexpr uint8_t kBrickSkip = 0, kBrickFree = 1, kBrickExact = 2;
 
 struct SweepArgs {
   void* vox;
-  uint32_t* spans;
+  uint32_t* count;  // number of entries in list
+  uint32_t* list;   // bricks to visit: bx << 24 | by << 16 | bz << 8 | class
+  int nb;           // bricks per axis
+  float2* t8;       // raylen (min, max) per 8x8 tile, w8 per row
+  float2* t32;      // per 32x32 tile, w32 per row
+  int w8, w32;
   const float* raylen;
   const kf_pose* pose;
   unsigned long long* updated;
@@ -47,73 +62,221 @@ __global__ __launch_bounds__(256) void volume_fill(uint4* p, size_t n16, uint4 w
   for (; i < n16; i += stride) p[i] = word;
 }
 
-// Conservative frustum span per column (FP64).  Margins: 0.05 px in the image
-// plane and 1e-4 m in depth, each far above the FP32 error of the per-voxel path
-// (< 3e-3 px, < 2e-6 m), plus one voxel on each side.
-__global__ __launch_bounds__(256) void column_spans(SweepArgs a) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.n * a.n) return;
-  const int ix = c / a.n, iy = c - ix * a.n;
-  const kf_posef P = a.pose->cw;
-  const float wxf = a.ox + ((float)ix + 0.5f) * a.h;
-  const float wyf = a.oy + ((float)iy + 0.5f) * a.h;
-  const double wx = wxf, wy = wyf;
-  const double wz0 = (double)a.oz + 0.5 * (double)a.h;
-  const double h = a.h;
-  double A[3], D[3];
+// Raylen extrema of the 8x8 and 32x32 pixel tiles (x = min, y = max).  Pixels
+// outside the image are ignored; invalid pixels (raylen 0) count, so a tile with
+// one has min 0.  One 256-thread block per 32x32 tile, a 2x2 quad per thread.
+__global__ __launch_bounds__(256) void raylen_tiles(const float* rl, int W, int H, float2* t8, int w8,
+                                                    float2* t32, int w32, uint32_t* list_count) {
+  __shared__ float qmin[16][16], qmax[16][16], s8[2][16];
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *list_count = 0;  // the next kernel fills it
+  const int qx = threadIdx.x & 15, qy = threadIdx.x >> 4;
+  const int x0 = blockIdx.x * 32 + 2 * qx, y0 = blockIdx.y * 32 + 2 * qy;
+  float mn = 3.0e38f, mx = 0.0f;
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    A[r] = (double)P.r[3 * r] * wx + (double)P.r[3 * r + 1] * wy + (double)P.r[3 * r + 2] * wz0 +
-           (double)P.t[r];
-    D[r] = (double)P.r[3 * r + 2] * h;
-  }
-  const double eps_px = 0.05, eps_z = 1e-4;
-  double lo = 0.0, hi = (double)(a.n - 1);
-  bool empty = false;
-  auto clip = [&](double c0, double c1) {
-    if (c1 > 0.0) {
-      lo = fmax(lo, -c0 / c1);
-    } else if (c1 < 0.0) {
-      hi = fmin(hi, -c0 / c1);
-    } else if (c0 < 0.0) {
-      empty = true;
+  for (int d = 0; d < 4; ++d) {
+    const int x = x0 + (d & 1), y = y0 + (d >> 1);
+    if (x < W && y < H) {
+      const float r = rl[y * W + x];
+      mn = fminf(mn, r);
+      mx = fmaxf(mx, r);
     }
-  };
-  clip(A[2] - ((double)a.dmin - eps_z), D[2]);
-  clip(((double)a.dmax + eps_z) - A[2], -D[2]);
-  const double ulo = 1.5 - eps_px, uhi = (double)a.W - 2.5 + eps_px;
-  const double vlo = 1.5 - eps_px, vhi = (double)a.H - 2.5 + eps_px;
-  const double fx = a.fx, fy = a.fy, cx = a.cx, cy = a.cy;
-  clip(fx * A[0] + (cx - ulo) * A[2], fx * D[0] + (cx - ulo) * D[2]);
-  clip((uhi - cx) * A[2] - fx * A[0], (uhi - cx) * D[2] - fx * D[0]);
-  clip(fy * A[1] + (cy - vlo) * A[2], fy * D[1] + (cy - vlo) * D[2]);
-  clip((vhi - cy) * A[2] - fy * A[1], (vhi - cy) * D[2] - fy * D[1]);
-  uint32_t span = 0x0000ffffu;  // z1 < z0: empty
-  if (!empty && lo <= hi + 2.0 && lo <= (double)a.n && hi >= -1.0) {
-    const int z0 = (int)fmax(0.0, floor(lo) - 1.0);
-    const int z1 = (int)fmin((double)(a.n - 1), ceil(hi) + 1.0);
-    if (z0 <= z1) span = (uint32_t)z0 | ((uint32_t)z1 << 16);
   }
-  a.spans[c] = span;
+  qmin[qy][qx] = mn;
+  qmax[qy][qx] = mx;
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const int i = threadIdx.x & 3, j = threadIdx.x >> 2;  // 8x8 tile (i, j) of the 32x32 tile
+    float a = 3.0e38f, b = 0.0f;
+    for (int v = 0; v < 4; ++v)
+      for (int u = 0; u < 4; ++u) {
+        a = fminf(a, qmin[4 * j + v][4 * i + u]);
+        b = fmaxf(b, qmax[4 * j + v][4 * i + u]);
+      }
+    s8[0][threadIdx.x] = a;
+    s8[1][threadIdx.x] = b;
+    const int tx = blockIdx.x * 4 + i, ty = blockIdx.y * 4 + j;
+    if (tx * 8 < W && ty * 8 < H) t8[ty * w8 + tx] = make_float2(a, b);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float a = 3.0e38f, b = 0.0f;
+    for (int q = 0; q < 16; ++q) {
+      a = fminf(a, s8[0][q]);
+      b = fmaxf(b, s8[1][q]);
+    }
+    t32[blockIdx.y * w32 + blockIdx.x] = make_float2(a, b);
+  }
 }
 
-// Exact per-voxel test; returns true and the new normalised sample f if updated.
-__device__ __forceinline__ bool voxel_sample(const SweepArgs& a, float ax, float ay, float az,
-                                             const float* r, int k, float& f) {
+// Raylen extrema over the pixels [u0, u1] x [v0, v1] (a superset: whole tiles), from
+// the 8x8 tiles when the range is small, else the 32x32 ones; false when it is too
+// large for either.
+__device__ __forceinline__ bool tile_range(const SweepArgs& a, int u0, int u1, int v0, int v1, float& mn,
+                                           float& mx) {
+  int sh;
+  const float2* t;
+  int w;
+  if (u1 - u0 <= 40 && v1 - v0 <= 40) {
+    sh = 3;
+    t = a.t8;
+    w = a.w8;
+  } else if (u1 - u0 <= 200 && v1 - v0 <= 200) {
+    sh = 5;
+    t = a.t32;
+    w = a.w32;
+  } else {
+    return false;
+  }
+  mn = 3.0e38f;
+  mx = 0.0f;
+  for (int ty = v0 >> sh; ty <= (v1 >> sh); ++ty)
+    for (int tx = u0 >> sh; tx <= (u1 >> sh); ++tx) {
+      const float2 e = t[ty * w + tx];
+      mn = fminf(mn, e.x);
+      mx = fmaxf(mx, e.y);
+    }
+  return true;
+}
+
+// Sweep class of each 8^3 brick (one thread per brick).  The brick's voxel centres
+// span the box of its eight corner centres; camera coordinates are affine in the
+// voxel position, so depth and distance extremes lie at corners and the image of
+// the box (all corners in front of the camera) is the convex hull of the corner
+// images.  With the FP32 pose and grid that the per-voxel test uses, evaluated in
+// FP64:
+//   skip : all corners before min_depth or beyond max_depth, or the footprint
+//          misses the updatable pixels (rounded u in [2, W-3], v in [2, H-3]), or
+//          the nearest point of the box is farther than the largest raylen under
+//          the footprint plus the band: no voxel is updated (the per-voxel test
+//          rejects it on depth, image bounds, an invalid pixel or sdf < -T);
+//   free : every corner inside the depth range, the footprint inside the
+//          updatable pixels, every raylen under it valid, and the farthest corner
+//          nearer than the smallest raylen minus the band: every voxel is updated
+//          with f = 1 (the test's free-space shortcut);
+//   exact: otherwise.
+// Margins: 0.05 px, 1e-4 m in depth, 2e-4 m in distance, against FP32 errors of the
+// per-voxel path below 4e-3 px and 2e-6 m.
+__device__ __forceinline__ uint8_t classify_brick(const SweepArgs& a, int bx, int by, int bz) {
+  const kf_posef P = a.pose->cw;
+  double R[9], t[3];
+  for (int q = 0; q < 9; ++q) R[q] = (double)P.r[q];
+  for (int q = 0; q < 3; ++q) t[q] = (double)P.t[q];
+  const double h = (double)a.h;
+  const double lo[3] = {(double)a.ox + (8.0 * bx + 0.5) * h, (double)a.oy + (8.0 * by + 0.5) * h,
+                        (double)a.oz + (8.0 * bz + 0.5) * h};
+  const double hi[3] = {lo[0] + 7.0 * h, lo[1] + 7.0 * h, lo[2] + 7.0 * h};
+  double pzmin = 1e300, pzmax = -1e300, umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300, d2max = 0.0;
+  for (int c = 0; c < 8; ++c) {
+    const double wx = (c & 4) ? hi[0] : lo[0], wy = (c & 2) ? hi[1] : lo[1], wz = (c & 1) ? hi[2] : lo[2];
+    const double px = (R[0] * wx + R[1] * wy) + R[2] * wz + t[0];
+    const double py = (R[3] * wx + R[4] * wy) + R[5] * wz + t[1];
+    const double pz = (R[6] * wx + R[7] * wy) + R[8] * wz + t[2];
+    pzmin = fmin(pzmin, pz);
+    pzmax = fmax(pzmax, pz);
+    d2max = fmax(d2max, px * px + py * py + pz * pz);
+    if (pz > 1e-3) {
+      const double u = (double)a.fx * px / pz + (double)a.cx, v = (double)a.fy * py / pz + (double)a.cy;
+      umin = fmin(umin, u);
+      umax = fmax(umax, u);
+      vmin = fmin(vmin, v);
+      vmax = fmax(vmax, v);
+    }
+  }
+  const double mz = 1e-4, md = 2e-4, mpx = 0.05;
+  uint8_t cls = kBrickExact;
+  if (pzmax < (double)a.dmin - mz || pzmin > (double)a.dmax + mz) {
+    cls = kBrickSkip;
+  } else if (pzmin > 1e-3) {
+    umin -= mpx;
+    umax += mpx;
+    vmin -= mpx;
+    vmax += mpx;
+    const double ulo = 1.5, uhi = (double)a.W - 2.5, vlo = 1.5, vhi = (double)a.H - 2.5;
+    if (umax < ulo || umin >= uhi || vmax < vlo || vmin >= vhi) {
+      cls = kBrickSkip;
+    } else {
+      const int pu0 = (int)fmax(2.0, floor(umin + 0.5)), pu1 = (int)fmin((double)(a.W - 3), floor(umax + 0.5));
+      const int pv0 = (int)fmax(2.0, floor(vmin + 0.5)), pv1 = (int)fmin((double)(a.H - 3), floor(vmax + 0.5));
+      float rmin, rmax;
+      if (pu0 <= pu1 && pv0 <= pv1 && tile_range(a, pu0, pu1, pv0, pv1, rmin, rmax)) {
+        const bool inside = umin >= ulo && umax < uhi && vmin >= vlo && vmax < vhi;
+        // nearest point of the box to the camera centre -R^T t
+        double dn2 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          const double ck = -(R[k] * t[0] + R[3 + k] * t[1] + R[6 + k] * t[2]);
+          const double e = ck < lo[k] ? lo[k] - ck : (ck > hi[k] ? ck - hi[k] : 0.0);
+          dn2 += e * e;
+        }
+        const double band = (double)a.band;
+        if (inside && pzmin >= (double)a.dmin + mz && pzmax <= (double)a.dmax - mz && rmin > 0.0f &&
+            sqrt(d2max) + md < (double)rmin - band)
+          cls = kBrickFree;
+        else if (sqrt(dn2) - md > (double)rmax + band)
+          cls = kBrickSkip;
+      }
+    }
+  }
+  return cls;
+}
+
+// One thread per brick; free and exact bricks are appended to the work list (one
+// atomic per block; entries in brick order within a block).
+__global__ __launch_bounds__(256) void brick_classify(SweepArgs a) {
+  __shared__ uint32_t s_cnt[4], s_base;
+  const int nb = a.nb;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int bx = b / (nb * nb), r = b - bx * (nb * nb), by = r / nb, bz = r - by * nb;
+  const uint8_t cls = b < nb * nb * nb ? classify_brick(a, bx, by, bz) : kBrickSkip;
+  const bool keep = cls != kBrickSkip;
+  const unsigned long long m = __ballot(keep);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+  if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t tot = (s_cnt[0] + s_cnt[1]) + (s_cnt[2] + s_cnt[3]);
+    s_base = tot ? atomicAdd(a.count, tot) : 0u;
+  }
+  __syncthreads();
+  if (keep) {
+    uint32_t off = s_base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wave; ++w) off += s_cnt[w];
+    a.list[off] = ((uint32_t)bx << 24) | ((uint32_t)by << 16) | ((uint32_t)bz << 8) | cls;
+  }
+}
+
+// Exact per-voxel test in two parts, so that a wave can issue the raylen loads of
+// several chunks before using any.  voxel_project: camera point of voxel z of the
+// column, depth gate, projection with round half up and image bounds; returns the
+// pixel index (-1: not updated) and d2 = |p|^2.  voxel_finish: the test at that
+// pixel; true and the new normalised sample f if updated.
+struct VoxProj {
+  float d2;
+  int pix;
+};
+
+__device__ __forceinline__ VoxProj voxel_project(const SweepArgs& a, float ax, float ay, float az, const float* r,
+                                                 int k) {
+  VoxProj q;
+  q.pix = -1;
+  q.d2 = 0.0f;
   const float wz = a.oz + ((float)k + 0.5f) * a.h;
   const float px = ax + r[2] * wz;
   const float py = ay + r[5] * wz;
   const float pz = az + r[8] * wz;
-  if (!(pz >= a.dmin && pz <= a.dmax)) return false;
+  if (!(pz >= a.dmin && pz <= a.dmax)) return q;
   const float iz = 1.0f / pz;
   const float u = (a.fx * px + a.cx * pz) * iz;
   const float v = (a.fy * py + a.cy * pz) * iz;
   const float ru = round_half_up(u);
   const float rv = round_half_up(v);
-  if (!(ru >= 2.0f && ru < (float)(a.W - 2) && rv >= 2.0f && rv < (float)(a.H - 2))) return false;
-  const float rl = a.raylen[(int)rv * a.W + (int)ru];
+  if (!(ru >= 2.0f && ru < (float)(a.W - 2) && rv >= 2.0f && rv < (float)(a.H - 2))) return q;
+  q.pix = (int)rv * a.W + (int)ru;
+  q.d2 = (px * px + py * py) + pz * pz;
+  return q;
+}
+
+__device__ __forceinline__ bool voxel_finish(const SweepArgs& a, float rl, float d2, float& f) {
   if (!(rl > 0.0f)) return false;
-  const float d2 = (px * px + py * py) + pz * pz;
   // Outside |rl - dist| <= band the outcome follows from d2 alone: the margin
   // band - T >= max(0.01 T, 0.1 mm) is far above the FP32 error of d2, lo^2, hi^2,
   // dist and sdf (< 2 um for rays up to 10 m), so free space yields exactly f = 1
@@ -132,75 +295,118 @@ __device__ __forceinline__ bool voxel_sample(const SweepArgs& a, float ax, float
   return true;
 }
 
+// Marks brick (bx, by, bz) and its super-brick in the occupancy map (only when the
+// bits are missing).
+__device__ __forceinline__ void occ_mark_brick(const OccMap& occ, int bx, int by, int bz) {
+  unsigned long long* w = occ.words + (bx * occ.nb + by);
+  const unsigned long long bit = 1ull << bz;
+  if (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit) return;
+  __hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned* sw = occ.super + ((bx >> 2) * occ.nsb + (by >> 2));
+  const unsigned sbit = 1u << (bz >> 2);
+  if (!(__hip_atomic_load(sw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & sbit))
+    __hip_atomic_fetch_or(sw, sbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The sweep over the listed bricks.  Lane (yy, zz) = (lane >> 3, lane & 7) holds
+// voxel (8 bx + xs, 8 by + yy, 8 bz + zz) of each x-slice xs; the eight slices'
+// loads (and, for exact bricks, their raylen loads) are issued together.
 template <int FMT>
 __global__ __launch_bounds__(256) void integrate(SweepArgs a) {
+  using Word = typename std::conditional<FMT == KF_VOXEL_F16, uint32_t, float2>::type;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+  const int yy = lane >> 3, zz = lane & 7;
   const kf_posef P = a.pose->cw;
-  const int ncol = a.n * a.n;
+  const uint32_t items = *a.count;
+  const size_t n = (size_t)a.n, slice = n * n;
   int count = 0;
-  // Waves stride over the columns; neighbouring waves take neighbouring columns.
-  for (int c = blockIdx.x * kWavesPerBlock + wave; c < ncol; c += gridDim.x * kWavesPerBlock) {
-    const uint32_t span = a.spans[c];
-    const int z0 = (int)(span & 0xffffu), z1 = (int)(span >> 16);
-    if (z1 < z0) continue;
-    const int ix = c / a.n, iy = c - ix * a.n;
-    const float wx = a.ox + ((float)ix + 0.5f) * a.h;
-    const float wy = a.oy + ((float)iy + 0.5f) * a.h;
-    const float ax = (P.r[0] * wx + P.r[1] * wy) + P.t[0];
-    const float ay = (P.r[3] * wx + P.r[4] * wy) + P.t[1];
-    const float az = (P.r[6] * wx + P.r[7] * wy) + P.t[2];
-    const size_t col = (size_t)c * (size_t)a.n;
-    unsigned long long occ_bits = 0ull;  // z-bricks receiving f != 1 from this lane
-    // One voxel per lane per step: a wave covers 64 consecutive z (256 B of FP16
-    // or 512 B of FP32 voxels, fully coalesced), so few lanes idle at span ends.
-    for (int zb = z0; zb <= z1; zb += kWave) {
-      const int z = zb + lane;
-      float fs = 0.0f;
-      const bool upd = z <= z1 && voxel_sample(a, ax, ay, az, P.r, z, fs);
-      if (upd) {
-        ++count;
-        const size_t idx = col + (size_t)z;
-        if (FMT == KF_VOXEL_F16) {
-          uint32_t* p = static_cast<uint32_t*>(a.vox) + idx;
-          const uint32_t w = *p;
-          const float fo = half_bits_to_float(w & 0xffffu);
-          const float wo = (float)(w >> 16);
-          // Free space (f_old = f = 1) averages to exactly 1: (w + 1) / (w + 1).
-          const float fn = (fo == 1.0f && fs == 1.0f) ? 1.0f : (fo * wo + fs) / (wo + 1.0f);
-          const float wn = fminf(wo + 1.0f, a.maxw);
-          const uint32_t nw = float_to_half_bits(fn) | ((uint32_t)wn << 16);
-          if (nw != w) *p = nw;  // saturated free space (f = 1, w = max) is left as it is
-          if (a.occ.words && (nw & 0xffffu) != 0x3C00u) occ_bits |= occ_zbits(z);
-        } else {
-          float2* p = static_cast<float2*>(a.vox) + idx;
-          const float2 v = *p;
-          const float fn = (v.x == 1.0f && fs == 1.0f) ? 1.0f : (v.x * v.y + fs) / (v.y + 1.0f);
-          const float wn = fminf(v.y + 1.0f, a.maxw);
-          if (__float_as_uint(fn) != __float_as_uint(v.x) || __float_as_uint(wn) != __float_as_uint(v.y))
-            *p = make_float2(fn, wn);
-          if (a.occ.words && fn != 1.0f) occ_bits |= occ_zbits(z);
-        }
+  for (uint32_t i = blockIdx.x * kWavesPerBlock + wave; i < items; i += gridDim.x * kWavesPerBlock) {
+    const uint32_t e = a.list[i];
+    const int bx = (int)(e >> 24), by = (int)((e >> 16) & 255u), bz = (int)((e >> 8) & 255u);
+    const bool exact = (e & 255u) == kBrickExact;  // uniform
+    const int y = 8 * by + yy, z = 8 * bz + zz;
+    Word* base = static_cast<Word*>(a.vox) + ((size_t)(8 * bx) * n + (size_t)y) * n + (size_t)z;
+    bool upd[8];
+    float fs[8], rl[8];
+    VoxProj q[8];
+#pragma unroll
+    for (int xs = 0; xs < 8; ++xs) {
+      fs[xs] = 1.0f;
+      q[xs].pix = -1;
+      q[xs].d2 = 0.0f;
+    }
+    if (exact) {
+      const float wy = a.oy + ((float)y + 0.5f) * a.h;
+#pragma unroll
+      for (int xs = 0; xs < 8; ++xs) {
+        const float wx = a.ox + ((float)(8 * bx + xs) + 0.5f) * a.h;
+        const float ax = (P.r[0] * wx + P.r[1] * wy) + P.t[0];
+        const float ay = (P.r[3] * wx + P.r[4] * wy) + P.t[1];
+        const float az = (P.r[6] * wx + P.r[7] * wy) + P.t[2];
+        q[xs] = voxel_project(a, ax, ay, az, P.r, z);
       }
-      if (a.mask) {  // 64 consecutive voxels -> up to three 32-bit mask words
-        const unsigned long long m = __ballot(upd);
-        if (m) {
-          const size_t base = col + (size_t)zb;
-          const int sh = (int)(base & 31);
-          const size_t w0 = base >> 5;
-          if (lane == 0) {
-            const uint32_t v0 = (uint32_t)(m << sh);
-            if (v0) atomicOr(&a.mask[w0], v0);
-          } else if (lane == 1) {
-            const uint32_t v1 = (uint32_t)(m >> (32 - sh));
-            if (v1) atomicOr(&a.mask[w0 + 1], v1);
-          } else if (lane == 2 && sh > 0) {
-            const uint32_t v2 = (uint32_t)(m >> (64 - sh));
-            if (v2) atomicOr(&a.mask[w0 + 2], v2);
-          }
+    }
+    // one batch of loads (no branches: lanes without a pixel read pixel 0 and ignore it)
+    Word wv[8];
+#pragma unroll
+    for (int xs = 0; xs < 8; ++xs) {
+      rl[xs] = a.raylen[q[xs].pix >= 0 ? q[xs].pix : 0];
+      wv[xs] = base[xs * slice];
+    }
+    bool nz_any = false, nz_x0 = false;  // f != 1 written: anywhere / in slice 0
+#pragma unroll
+    for (int xs = 0; xs < 8; ++xs) {
+      upd[xs] = !exact || (q[xs].pix >= 0 && voxel_finish(a, rl[xs], q[xs].d2, fs[xs]));
+      if (!upd[xs]) continue;
+      ++count;
+      bool nz;
+      if (FMT == KF_VOXEL_F16) {
+        const uint32_t w = reinterpret_cast<const uint32_t&>(wv[xs]);
+        const float fo = half_bits_to_float(w & 0xffffu);
+        const float wo = (float)(w >> 16);
+        // Free space (f_old = f = 1) averages to exactly 1: (w + 1) / (w + 1).
+        const float fn = (fo == 1.0f && fs[xs] == 1.0f) ? 1.0f : (fo * wo + fs[xs]) / (wo + 1.0f);
+        const float wn = fminf(wo + 1.0f, a.maxw);
+        const uint32_t nw = float_to_half_bits(fn) | ((uint32_t)wn << 16);
+        // saturated free space (f = 1, w = max) is left as it is
+        if (nw != w) reinterpret_cast<uint32_t*>(base)[xs * slice] = nw;
+        nz = (nw & 0xffffu) != 0x3C00u;
+      } else {
+        const float2 v = reinterpret_cast<const float2&>(wv[xs]);
+        const float fn = (v.x == 1.0f && fs[xs] == 1.0f) ? 1.0f : (v.x * v.y + fs[xs]) / (v.y + 1.0f);
+        const float wn = fminf(v.y + 1.0f, a.maxw);
+        if (__float_as_uint(fn) != __float_as_uint(v.x) || __float_as_uint(wn) != __float_as_uint(v.y))
+          reinterpret_cast<float2*>(base)[xs * slice] = make_float2(fn, wn);
+        nz = fn != 1.0f;
+      }
+      nz_any = nz_any || nz;
+      if (xs == 0) nz_x0 = nz;
+    }
+    if (a.mask) {  // per slice: eight rows of eight z-bits, each inside one 32-bit word
+#pragma unroll
+      for (int xs = 0; xs < 8; ++xs) {
+        const unsigned long long m = __ballot(upd[xs]);
+        const uint32_t bits = (uint32_t)(m >> (8 * yy)) & 0xffu;
+        if (zz == 0 && bits) {
+          const size_t v0 = ((size_t)(8 * bx + xs) * n + (size_t)y) * n + (size_t)(8 * bz);
+          atomicOr(&a.mask[v0 >> 5], bits << (v0 & 31));
         }
       }
     }
-    if (a.occ.words) occ_mark_column(a.occ, ix, iy, occ_bits, lane);
+    if (a.occ.words) {
+      // A clear brick bit promises f == 1 on the brick extended by one voxel along +x,
+      // +y, +z: a voxel with f != 1 marks its brick and, on the low face of an axis,
+      // the neighbour below along that axis.
+      const unsigned long long A = __ballot(nz_any), X = __ballot(nz_x0);
+      if (A) {  // uniform
+        const unsigned long long Y0 = 0xffull, Z0 = 0x0101010101010101ull;  // lanes yy == 0 / zz == 0
+        if (lane < 8) {
+          const int dx = lane & 1, dy = (lane >> 1) & 1, dz = lane >> 2;
+          const unsigned long long src = (dx ? X : A) & (dy ? Y0 : ~0ull) & (dz ? Z0 : ~0ull);
+          if (src && bx >= dx && by >= dy && bz >= dz) occ_mark_brick(a.occ, bx - dx, by - dy, bz - dz);
+        }
+      }
+    }
   }
   if (a.updated) {  // one device atomic per block (uniform branch)
     __shared__ int red[kWavesPerBlock];
@@ -372,7 +578,13 @@ kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integ
   SweepArgs a;
   const kf_volume_desc& d = v->desc;
   a.vox = v->d_voxels;
-  a.spans = v->d_spans;
+  a.count = v->d_bricks;
+  a.list = v->d_bricks + 1;
+  a.nb = d.n / 8;
+  a.w8 = div_up(k0.width, 8);
+  a.w32 = div_up(k0.width, 32);
+  a.t8 = v->d_rtiles;
+  a.t32 = v->d_rtiles + kRayTiles8;
   a.raylen = d_raylen;
   a.pose = d_pose;
   a.updated = d_updated;
@@ -395,11 +607,15 @@ kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integ
   a.inv_trunc = 1.0f / d.trunc_m;
   a.band = d.trunc_m + std::max(0.01f * d.trunc_m, 1e-4f);
   a.maxw = d.max_weight;
-  const int cols = d.n * d.n;
-  hipLaunchKernelGGL(column_spans, dim3(div_up(cols, 256)), dim3(256), 0, s, a);
-  KF_TRY(check_launch("column_spans"));
+  KF_CHECK_ARG(k0.width <= kMaxImage && k0.height <= kMaxImage, "image too large");
+  hipLaunchKernelGGL(raylen_tiles, dim3(div_up(k0.width, 32), div_up(k0.height, 32)), dim3(256), 0, s, d_raylen,
+                     k0.width, k0.height, a.t8, a.w8, a.t32, a.w32, a.count);
+  KF_TRY(check_launch("raylen_tiles"));
+  const int nbricks = a.nb * a.nb * a.nb;
+  hipLaunchKernelGGL(brick_classify, dim3(div_up(nbricks, 256)), dim3(256), 0, s, a);
+  KF_TRY(check_launch("brick_classify"));
   if (between) KF_HIP(hipEventRecord(between, s));
-  const int blocks = std::min(div_up(cols, kWavesPerBlock), kSweepMaxBlocks);
+  const int blocks = std::min(div_up(nbricks, kWavesPerBlock), kSweepListBlocks);
   if (d.format == KF_VOXEL_F16)
     hipLaunchKernelGGL(integrate<KF_VOXEL_F16>, dim3(blocks), dim3(256), 0, s, a);
   else
