@@ -88,14 +88,17 @@ struct kf_tracker {
 
 namespace kf {
 
-// Launchers implemented in the .hip translation units.
+// Launchers implemented in the .hip translation units.  kernel_start / kernel_stop
+// (optional, together): events that take the main kernel's own start and end
+// times (hipExtLaunchKernelGGL).
 kf_status launch_preprocess(const kf_intrinsics& k0, const kf_preprocess_params& p,
                             const uint16_t* d_depth, kf_frame_maps& live, hipStream_t s);
 kf_status launch_volume_reset(kf_volume* v, hipStream_t s);
 kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integrate_params& p,
                            const float* d_raylen, const kf_pose* d_pose,
                            unsigned long long* d_updated, uint32_t* d_mask, hipStream_t s,
-                           hipEvent_t between = nullptr);
+                           hipEvent_t between = nullptr, hipEvent_t kernel_start = nullptr,
+                           hipEvent_t kernel_stop = nullptr);
 kf_status launch_volume_stats(const kf_volume* v, double* d_out, hipStream_t s);
 // Recomputes the occupancy map from the voxels when it is stale (no-op otherwise).
 kf_status ensure_occupancy(const kf_volume* v, hipStream_t s);
@@ -107,7 +110,8 @@ constexpr size_t kRayTiles = kRayTiles8 + (size_t)(kMaxImage / 32) * (kMaxImage 
 kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_raycast_params& p,
                          const kf_pose* d_pose, kf_frame_maps& model, unsigned long long* d_steps,
                          hipStream_t s, hipEvent_t after_raycast = nullptr,
-                         unsigned long long* d_diag = nullptr);
+                         unsigned long long* d_diag = nullptr, hipEvent_t kernel_start = nullptr,
+                         hipEvent_t kernel_stop = nullptr);
 // single_launch: the whole call as one launch (also when the tracker's parameters
 // do not ask for it); used where most calls are skipped by the statistics' flag.
 kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_frame_maps& model,

@@ -53,6 +53,7 @@ struct kf_pipeline {
   hipGraphExec_t exec = nullptr;
   bool stage_timing = false;
   hipEvent_t ev[KF_STAGES + 1] = {};
+  hipEvent_t kev[4] = {};  // own start / end of the sweep kernel, then of the raycast kernel
   float stage_ms[KF_STAGES] = {};
   // stage-timing mode: [updated voxels, raycast steps, raycast samples that loaded
   // corners, raycast batches, raycast steps skipped through free space, sum over
@@ -260,10 +261,11 @@ kf_status record_track_frame(kf_pipeline* p, bool timed) {
   }
   if (timed) KF_HIP(hipEventRecord(p->ev[2], s));
   KF_TRY(launch_integrate(p->vol, d.intr, d.integ, p->live.raylen, &p->d_poses[kCur], upd, nullptr, s,
-                          timed ? p->ev[3] : nullptr));
+                          timed ? p->ev[3] : nullptr, timed ? p->kev[0] : nullptr, timed ? p->kev[1] : nullptr));
   if (timed) KF_HIP(hipEventRecord(p->ev[4], s));
   KF_TRY(launch_raycast(p->vol, d.intr, d.ray, &p->d_poses[kCur], p->model, steps, s, timed ? p->ev[5] : nullptr,
-                        timed ? &p->d_counters[2] : nullptr));
+                        timed ? &p->d_counters[2] : nullptr, timed ? p->kev[2] : nullptr,
+                        timed ? p->kev[3] : nullptr));
   hipLaunchKernelGGL(pose_shift, dim3(1), dim3(64), 0, s, p->d_poses, p->d_traj, p->d_traj_count);
   KF_TRY(check_launch("pose_shift"));
   if (timed) KF_HIP(hipEventRecord(p->ev[6], s));
@@ -350,6 +352,8 @@ extern "C" kf_status kf_pipeline_destroy(kf_pipeline* p) {
   if (p->own_stream) (void)hipStreamDestroy(p->stream);
   for (hipEvent_t& e : p->ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t& e : p->kev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipFree(p->d_counters);
   (void)hipFree(p->d_traj);
   (void)hipFree(p->d_traj_count);
@@ -398,6 +402,7 @@ extern "C" kf_status kf_pipeline_create(const kf_pipeline_desc* d, kf_stream s, 
     if (e == hipSuccess) e = hipMemset(p->d_counters, 0, kCounters * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(p->d_stats, 0, 2 * sizeof(kf_icp_stats));
     for (int i = 0; i < KF_STAGES + 1 && e == hipSuccess; ++i) e = hipEventCreate(&p->ev[i]);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&p->kev[i]);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) st = hip_fail(e, "pipeline allocation");
   }
@@ -438,6 +443,9 @@ extern "C" kf_status kf_pipeline_process(kf_pipeline* p, const uint16_t* d_depth
       KF_HIP(hipEventSynchronize(p->ev[KF_STAGES]));
       for (int i = 0; i < KF_STAGES; ++i)
         KF_HIP(hipEventElapsedTime(&p->stage_ms[i], p->ev[i], p->ev[i + 1]));
+      // the sweep and raycast stages are their kernels' own start-to-end times
+      KF_HIP(hipEventElapsedTime(&p->stage_ms[3], p->kev[0], p->kev[1]));
+      KF_HIP(hipEventElapsedTime(&p->stage_ms[4], p->kev[2], p->kev[3]));
     }
   } else {
     if (!p->exec) {

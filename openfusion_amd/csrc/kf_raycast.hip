@@ -12,6 +12,8 @@
 // 8x8 tile (rays in a wave stay coherent in the volume).  Block ids are remapped
 // so each of the 8 XCDs (which receive blocks round-robin) owns a contiguous
 // band of image rows and its L2 sees neighbouring rays.
+#include <hip/hip_ext.h>
+
 #include <type_traits>
 
 #include "kf_device.hpp"
@@ -432,7 +434,8 @@ __global__ __launch_bounds__(256) void subsample_maps(SubArgs s) {
 
 kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_raycast_params& p,
                          const kf_pose* d_pose, kf_frame_maps& model, unsigned long long* d_steps,
-                         hipStream_t s, hipEvent_t after_raycast, unsigned long long* d_diag) {
+                         hipStream_t s, hipEvent_t after_raycast, unsigned long long* d_diag,
+                         hipEvent_t kernel_start, hipEvent_t kernel_stop) {
   KF_CHECK_ARG(d_pose != nullptr, "pose is NULL");
   KF_CHECK_ARG(model.levels >= 1 && model.levels <= KF_MAX_LEVELS, "model levels must be 1..3");
   KF_CHECK_ARG(p.step_far > 0.0f && p.step_near > 0.0f && p.max_steps > 0, "bad march steps");
@@ -477,17 +480,12 @@ kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_r
   const int grid = ((nb + kXcds - 1) / kXcds) * kXcds;
   // 32-bit voxel offsets when every byte offset of the volume fits in 32 bits
   const bool off32 = v->bytes <= 0xffffffffull && d.n <= 1024;
-  if (d.format == KF_VOXEL_F16) {
-    if (off32)
-      hipLaunchKernelGGL((raycast<KF_VOXEL_F16, true>), dim3(grid), dim3(256), 0, s, a);
-    else
-      hipLaunchKernelGGL((raycast<KF_VOXEL_F16, false>), dim3(grid), dim3(256), 0, s, a);
-  } else {
-    if (off32)
-      hipLaunchKernelGGL((raycast<KF_VOXEL_F32, true>), dim3(grid), dim3(256), 0, s, a);
-    else
-      hipLaunchKernelGGL((raycast<KF_VOXEL_F32, false>), dim3(grid), dim3(256), 0, s, a);
-  }
+  auto kern = d.format == KF_VOXEL_F16 ? (off32 ? raycast<KF_VOXEL_F16, true> : raycast<KF_VOXEL_F16, false>)
+                                       : (off32 ? raycast<KF_VOXEL_F32, true> : raycast<KF_VOXEL_F32, false>);
+  if (kernel_start && kernel_stop)
+    hipExtLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, kernel_start, kernel_stop, 0, a);
+  else
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, a);
   KF_TRY(check_launch("raycast"));
   if (after_raycast) KF_HIP(hipEventRecord(after_raycast, s));
   if (model.levels > 1) {

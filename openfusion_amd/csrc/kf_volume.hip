@@ -21,6 +21,8 @@
 //      voxel.  A voxel is stored only if its bits change.  Skip, free and exact give
 //      each voxel the outcome of the per-voxel test, so voxels, update masks and
 //      counts are those of testing every voxel.
+#include <hip/hip_ext.h>
+
 #include <type_traits>
 
 #include "kf_device.hpp"
@@ -571,7 +573,8 @@ kf_status ensure_occupancy(const kf_volume* v, hipStream_t s) {
 
 kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integrate_params& p,
                            const float* d_raylen, const kf_pose* d_pose, unsigned long long* d_updated,
-                           uint32_t* d_mask, hipStream_t s, hipEvent_t between) {
+                           uint32_t* d_mask, hipStream_t s, hipEvent_t between, hipEvent_t kernel_start,
+                           hipEvent_t kernel_stop) {
   KF_CHECK_ARG(d_raylen && d_pose, "raylen/pose is NULL");
   KF_CHECK_ARG(p.min_depth_m >= 0.001f && p.max_depth_m > p.min_depth_m, "bad depth range");
   KF_CHECK_ARG(k0.width > 4 && k0.height > 4, "image too small");
@@ -616,10 +619,11 @@ kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integ
   KF_TRY(check_launch("brick_classify"));
   if (between) KF_HIP(hipEventRecord(between, s));
   const int blocks = std::min(div_up(nbricks, kWavesPerBlock), kSweepListBlocks);
-  if (d.format == KF_VOXEL_F16)
-    hipLaunchKernelGGL(integrate<KF_VOXEL_F16>, dim3(blocks), dim3(256), 0, s, a);
+  auto kern = d.format == KF_VOXEL_F16 ? integrate<KF_VOXEL_F16> : integrate<KF_VOXEL_F32>;
+  if (kernel_start && kernel_stop)
+    hipExtLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, kernel_start, kernel_stop, 0, a);
   else
-    hipLaunchKernelGGL(integrate<KF_VOXEL_F32>, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, a);
   return check_launch("integrate");
 }
 
