@@ -182,9 +182,24 @@ __device__ __forceinline__ void halve(float* s, float* c, bool upper) {
 // two per step with their loads in flight together), the 28 terms per thread,
 // the reduce-scatter wave reduction and the combine over the block's waves,
 // written as one slot: 28 (sum, error) pairs and the inlier / candidate counts.
+// The model samples of a thread's first two pixels in block_partial; they do not
+// depend on the pose, so a step can load them before it solves for the pose.
+struct PxPair {
+  PxModel q0, q1;
+};
+
+template <int WAVES>
+__device__ __forceinline__ PxPair prefetch_models(const IcpArgs& a, int bid, int nblk) {
+  constexpr int T = WAVES * kWave;
+  const int P = a.W * a.H, stride = nblk * T;
+  const int i0 = bid * T + (int)threadIdx.x, i1 = i0 + stride;
+  const int j0 = i0 < P ? i0 : 0;  // in bounds; unused when the block has no pixel here
+  return PxPair{load_model(a, j0), load_model(a, i1 < P ? i1 : j0)};
+}
+
 template <bool COMP, int WAVES>
 __device__ __forceinline__ void block_partial(const IcpArgs& a, const kf_posef& wc, const kf_posef& cw, int bid,
-                                              int nblk, float* slot) {
+                                              int nblk, float* slot, const PxPair* pre = nullptr) {
   __shared__ float red_s[WAVES][kPad];
   __shared__ float red_c[WAVES][kPad];
   __shared__ int red_n[WAVES][2];
@@ -199,11 +214,13 @@ __device__ __forceinline__ void block_partial(const IcpArgs& a, const kf_posef& 
   int n_in = 0, n_cand = 0;
   const int P = a.W * a.H;
   const int stride = nblk * T;
+  bool first = pre != nullptr;
   for (int i0 = bid * T + tid; i0 < P; i0 += 2 * stride) {
     const int i1 = i0 + stride;
     const bool has1 = i1 < P;
-    const PxModel q0 = load_model(a, i0);
-    const PxModel q1 = load_model(a, has1 ? i1 : i0);
+    const PxModel q0 = first ? pre->q0 : load_model(a, i0);
+    const PxModel q1 = first ? pre->q1 : load_model(a, has1 ? i1 : i0);
+    first = false;
     int r0, r1;
     const int j0 = project_model(a, cw, q0, r0);
     int j1 = project_model(a, cw, q1, r1);
@@ -339,14 +356,25 @@ __device__ __forceinline__ void sum_slots(const float* slots, int nb, double* su
   __syncthreads();
 }
 
+// Orders the LDS accesses of one wavefront around this point (a wave executes its
+// LDS instructions in order; this keeps the compiler from moving accesses across).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Solve (A^T A) x = A^T b from sum[30] and compose the Euler-ZYX increment onto
 // Tin (LDS, 16 doubles): Tout (LDS) and the FP32 fields wc / cw (LDS).  Returns
 // false for a singular system (fewer than kMinInliers inliers or a Cholesky pivot
 // not above 1e-9 x the largest diagonal); then the outputs are untouched.  With
 // `st`, thread 0 writes the level statistics and, when singular, status 1.
-// Cholesky is right-looking and lane-parallel: step j takes the pivot on lane 0,
-// the column below it on lanes j+1..5 and the trailing update on one lane per
-// entry; every entry receives the same subtractions in the same order as the
+// Called by the whole block; the first wavefront alone solves (its lanes stay in
+// step, so it needs no block barriers), the others wait at the final barrier.
+// Cholesky is right-looking and lane-parallel: at step j every lane takes the
+// pivot sqrt(W[j][j]) and its reciprocal itself, lanes j+1..5 write the column
+// below it and one lane per trailing entry subtracts the product of its two column
+// entries; every entry receives the same subtractions in the same order as the
 // serial left-looking loop (cholesky_solve6), so the factor has the same bits.
 // The substitutions are serial; the sines / cosines (3 lanes), the product (16
 // lanes) and the FP32 fields (24 lanes) are lane-parallel with the expressions of
@@ -357,94 +385,96 @@ __device__ __forceinline__ bool solve_compose(const double* sum, const double* T
   __shared__ double W[6][6], Lm[6][6], Li[6];
   __shared__ int tok;
   const int tid = threadIdx.x;
-  if (tid < 36) {
-    const int i = tid / 6, k = tid - 6 * (tid / 6);
-    const int r = i < k ? i : k, c = i < k ? k : i;
-    W[i][k] = sum[r * (11 - r) / 2 + c];
-  }
-  if (tid == 0) {
-    const double inl = sum[28], cand = sum[29];
+  if (tid < kWave) {
+    if (tid < 36) {
+      const int i = tid / 6, k = tid - 6 * (tid / 6);
+      const int r = i < k ? i : k, c = i < k ? k : i;
+      W[i][k] = sum[r * (11 - r) / 2 + c];
+    }
+    const double inl = sum[28];
     const int ni = (int)inl;
-    if (st) {
+    if (tid == 0 && st) {
       st->inliers = ni;
-      st->candidates = (int)cand;
+      st->candidates = (int)sum[29];
       st->residual_rms = ni > 0 ? sqrt(sum[27] / inl) : 0.0;
       st->level_residual[level] = st->residual_rms;
       st->level_inliers[level] = ni;
     }
-    tok = ni >= kMinInliers ? 1 : 0;
-  }
-  __syncthreads();
-  double thr = 0.0;
-  {
-    double dmax = 0.0;
-    for (int i = 0; i < 6; ++i) dmax = W[i][i] > dmax ? W[i][i] : dmax;
-    thr = kPivotRelEps * dmax;
-  }
-  for (int j = 0; j < 6 && tok; ++j) {  // tok is read from LDS after a barrier: uniform
-    if (tid == 0) {
+    bool ok = ni >= kMinInliers;  // uniform over the wave from here on
+    wave_sync();
+    double thr = 0.0;
+    {
+      double dmax = 0.0;
+      for (int i = 0; i < 6; ++i) dmax = W[i][i] > dmax ? W[i][i] : dmax;
+      thr = kPivotRelEps * dmax;
+    }
+    for (int j = 0; j < 6 && ok; ++j) {
       const double d = W[j][j];
       if (!(d > thr)) {
-        tok = 0;
-      } else {
-        const double ljj = sqrt(d);
+        ok = false;
+        break;
+      }
+      const double ljj = sqrt(d);
+      const double inv = 1.0 / ljj;
+      if (tid == j) {
         Lm[j][j] = ljj;
-        Li[j] = 1.0 / ljj;
+        Li[j] = inv;
+      } else if (tid > j && tid < 6) {
+        Lm[tid][j] = W[tid][j] * inv;
+      }
+      if (tid < 36) {
+        const int i = tid / 6, k = tid - 6 * (tid / 6);
+        if (k > j && i >= k) W[i][k] = W[i][k] - (W[i][j] * inv) * (W[k][j] * inv);
+      }
+      wave_sync();
+    }
+    if (tid == 0) {
+      if (ok) {
+        const double* g = sum + 21;
+        double y[6], x[6];
+        for (int i = 0; i < 6; ++i) {
+          double s = g[i];
+          for (int m = 0; m < i; ++m) s = s - Lm[i][m] * y[m];
+          y[i] = s * Li[i];
+        }
+        for (int i = 5; i >= 0; --i) {
+          double s = y[i];
+          for (int m = i + 1; m < 6; ++m) s = s - Lm[m][i] * x[m];
+          x[i] = s * Li[i];
+        }
+        for (int q = 0; q < 6; ++q) tx[q] = x[q];
+      } else if (st) {
+        st->status = 1;
+      }
+      tok = ok ? 1 : 0;
+    }
+    wave_sync();
+    if (ok) {
+      if (tid < 3) {
+        const double ang = -tx[tid];  // rotation increment is the negated solution
+        small_sincos(ang, &tcs[2 * tid], &tcs[2 * tid + 1]);
+      }
+      wave_sync();
+      if (tid < 16)
+        tD[tid] = euler_zyx_entry(tid, tcs[0], tcs[1], tcs[2], tcs[3], tcs[4], tcs[5], tx[3], tx[4], tx[5]);
+      wave_sync();
+      if (tid < 16) {
+        const int i = tid >> 2, j = tid & 3;
+        Tout[tid] = (i == 3) ? (j == 3 ? 1.0 : 0.0) : mat4_mul_entry(tD, Tin, i, j);
+      }
+      wave_sync();
+      if (tid < 12) {  // wc: R (9) and t (3)
+        if (tid < 9) wc->r[tid] = (float)Tout[(tid / 3) * 4 + tid % 3];
+        else wc->t[tid - 9] = (float)Tout[(tid - 9) * 4 + 3];
+      } else if (tid < 24) {  // cw = rigid inverse: R^T and -R^T t
+        const int q = tid - 12;
+        if (q < 9) cw->r[q] = (float)Tout[(q % 3) * 4 + q / 3];
+        else cw->t[q - 9] = (float)rigid_inverse_t(Tout, q - 9);
       }
     }
-    __syncthreads();
-    if (!tok) break;
-    if (tid > j && tid < 6) Lm[tid][j] = W[tid][j] * Li[j];
-    __syncthreads();
-    if (tid < 36) {
-      const int i = tid / 6, k = tid - 6 * (tid / 6);
-      if (k > j && i >= k) W[i][k] = W[i][k] - Lm[i][j] * Lm[k][j];
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    if (tok) {
-      const double* g = sum + 21;
-      double y[6], x[6];
-      for (int i = 0; i < 6; ++i) {
-        double s = g[i];
-        for (int m = 0; m < i; ++m) s = s - Lm[i][m] * y[m];
-        y[i] = s * Li[i];
-      }
-      for (int i = 5; i >= 0; --i) {
-        double s = y[i];
-        for (int m = i + 1; m < 6; ++m) s = s - Lm[m][i] * x[m];
-        x[i] = s * Li[i];
-      }
-      for (int q = 0; q < 6; ++q) tx[q] = x[q];
-    } else if (st) {
-      st->status = 1;
-    }
   }
   __syncthreads();
-  if (!tok) return false;  // uniform
-  if (tid < 3) {
-    const double ang = -tx[tid];  // rotation increment is the negated solution
-    small_sincos(ang, &tcs[2 * tid], &tcs[2 * tid + 1]);
-  }
-  __syncthreads();
-  if (tid < 16) tD[tid] = euler_zyx_entry(tid, tcs[0], tcs[1], tcs[2], tcs[3], tcs[4], tcs[5], tx[3], tx[4], tx[5]);
-  __syncthreads();
-  if (tid < 16) {
-    const int i = tid >> 2, j = tid & 3;
-    Tout[tid] = (i == 3) ? (j == 3 ? 1.0 : 0.0) : mat4_mul_entry(tD, Tin, i, j);
-  }
-  __syncthreads();
-  if (tid < 12) {  // wc: R (9) and t (3)
-    if (tid < 9) wc->r[tid] = (float)Tout[(tid / 3) * 4 + tid % 3];
-    else wc->t[tid - 9] = (float)Tout[(tid - 9) * 4 + 3];
-  } else if (tid < 24) {  // cw = rigid inverse: R^T and -R^T t
-    const int q = tid - 12;
-    if (q < 9) cw->r[q] = (float)Tout[(q % 3) * 4 + q / 3];
-    else cw->t[q - 9] = (float)rigid_inverse_t(Tout, q - 9);
-  }
-  __syncthreads();
-  return true;
+  return tok != 0;  // uniform
 }
 
 // Copies a pose (thread-parallel, 28 words of 8 bytes).
@@ -498,6 +528,7 @@ __global__ __launch_bounds__(kStepThreads) void icp_step(IcpArgs a, StepArgs S) 
     if (blockIdx.x == 0) copy_pose(S.pose_in, S.pose_out);
     return;
   }
+  const PxPair pre = prefetch_models<kStepThreads / kWave>(a, blockIdx.x, gridDim.x);
   if (S.first) {
     if (tid < 12) {
       if (tid < 9) s_wc.r[tid] = S.pose_in->wc.r[tid];
@@ -543,7 +574,7 @@ __global__ __launch_bounds__(kStepThreads) void icp_step(IcpArgs a, StepArgs S) 
   }
   __syncthreads();
   block_partial<COMP, kStepThreads / kWave>(a, s_wc, s_cw, blockIdx.x, gridDim.x,
-                                            S.slots_out + (size_t)blockIdx.x * kSlot);
+                                            S.slots_out + (size_t)blockIdx.x * kSlot, &pre);
 }
 
 // End of tracking (one block): solves the last step's slots, writes the output
