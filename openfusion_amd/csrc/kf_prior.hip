@@ -94,6 +94,10 @@ struct GemmArgs {
   int relu;
   const float* addend;
   uint16_t* Cb;
+  // Convolution operand (convX set): A(m, k) is the im2col element of a 3x3 /
+  // stride-2 / pad-1 convolution read from the BF16 NHWC input convX instead.
+  const uint16_t* convX;
+  int conv_cin, conv_hin, conv_win, conv_hout, conv_wout;
 };
 
 // Each 256-thread block computes a 32x32 tile of C (four 16x16 wave tiles on
@@ -169,6 +173,44 @@ __device__ __forceinline__ void stage(uint16_t (*T)[kPitch], const uint16_t* X, 
   }
 }
 
+// Stages A rows [row0, row0 + 32) of a convolution directly from its BF16 NHWC
+// input: k = (ky * 3 + kx) * cin + ci with cin a multiple of 8, so 8 consecutive k
+// are 8 channels of one tap (one 16-byte load, zero outside the image); the values
+// are those an im2col of the same input would hold.
+__device__ __forceinline__ void stage_conv(uint16_t (*T)[kPitch], const GemmArgs& g, int row0, int kn, int kpad) {
+  const int tid = threadIdx.x;
+  const int vpr = kpad >> 3;
+  const int total = 32 * vpr;
+  const int pix = g.conv_hout * g.conv_wout;
+  uint4 v[kVecPerThread];
+#pragma unroll
+  for (int u = 0; u < kVecPerThread; ++u) {
+    const int e = tid + 256 * u;
+    const int r = e / vpr, c = e - (e / vpr) * vpr;
+    const int m = row0 + r;
+    bool in = e < total && m < g.M && 8 * c < kn;
+    long off = 0;
+    if (in) {
+      const int n = m / pix, q = m - n * pix;
+      const int oy = q / g.conv_wout, ox = q - oy * g.conv_wout;
+      const int tap = (8 * c) / g.conv_cin, ci = 8 * c - tap * g.conv_cin;
+      const int ky = tap / 3, kx = tap - 3 * ky;
+      const int iy = 2 * oy + ky - 1, ix = 2 * ox + kx - 1;
+      in = iy >= 0 && iy < g.conv_hin && ix >= 0 && ix < g.conv_win;
+      off = (((long)n * g.conv_hin + iy) * g.conv_win + ix) * g.conv_cin + ci;
+    }
+    v[u] = in ? *reinterpret_cast<const uint4*>(g.convX + off) : make_uint4(0, 0, 0, 0);
+  }
+#pragma unroll
+  for (int u = 0; u < kVecPerThread; ++u) {
+    const int e = tid + 256 * u;
+    if (e < total) {
+      const int r = e / vpr, c = e - (e / vpr) * vpr;
+      *reinterpret_cast<uint4*>(&T[r][8 * c]) = v[u];
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
   __shared__ __align__(16) uint16_t As[32][kPitch];
   __shared__ __align__(16) uint16_t Bt[32][kPitch];
@@ -182,7 +224,10 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
   for (int kc0 = kbeg; kc0 < kend; kc0 += kKc) {
     const int kn = min(kKc, kend - kc0);
     const int kpad = (kn + 15) & ~15;
-    stage(As, g.A, g.sam, g.sak, mt, g.M, kc0, kn, kpad);
+    if (g.convX)  // uniform; the whole K is one chunk for every convolution
+      stage_conv(As, g, mt, kn, kpad);
+    else
+      stage(As, g.A, g.sam, g.sak, mt, g.M, kc0, kn, kpad);
     stage(Bt, g.B, g.sbn, g.sbk, nt, g.N, kc0, kn, kpad);
     __syncthreads();
     for (int kk = 0; kk < kpad; kk += 16) {
@@ -431,8 +476,15 @@ struct Epilogue {
 
 kf_status gemm(hipStream_t s, const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, float* C,
                long ldc, int M, int N, int K, bool accumulate, float* part = nullptr, long part_floats = 0,
-               const Epilogue& ep = Epilogue()) {
+               const Epilogue& ep = Epilogue(), const ConvShape* conv = nullptr, const uint16_t* convX = nullptr) {
   GemmArgs g;
+  g.convX = convX;
+  g.conv_cin = conv ? conv->cin : 0;
+  g.conv_hin = conv ? conv->hin : 0;
+  g.conv_win = conv ? conv->win : 0;
+  g.conv_hout = conv ? conv->hout : 0;
+  g.conv_wout = conv ? conv->wout : 0;
+  if (convX) KF_CHECK_ARG(conv->cin % 8 == 0 && K == conv->kc && K <= kKc, "implicit convolution shape");
   g.bias = ep.bias;
   g.relu = ep.relu;
   g.addend = ep.addend;
@@ -481,27 +533,35 @@ kf_status sync_weights(kf_prior* P, hipStream_t s) {
   return check_launch("to_bf16(weights)");
 }
 
-// out: where the twists go (P->O when NULL; training reads P->O).
+// out: where the twists go (P->O when NULL; training reads P->O).  cols: the
+// im2col matrices of every layer are materialised (training: the weight gradients
+// read them); without, layers 2-4 read their input directly in the GEMM (the
+// previous layer's epilogue leaves its BF16 copy in the next layer's im2col
+// buffer), with the same A values and hence the same outputs.
 kf_status forward(kf_prior* P, const float* frames, const int32_t* pairs, const float* prev, int B, hipStream_t s,
-                  float* out = nullptr) {
+                  float* out = nullptr, bool cols = true) {
   const ParamLayout& L = layout();
   for (int l = 0; l < kConvs; ++l) {
     const ConvShape& c = conv(l);
     const long M = (long)B * c.hout * c.wout;
-    if (l == 0)
+    if (l == 0) {
       hipLaunchKernelGGL(im2col_pairs, dim3(blocks_for(M * kcp(0))), dim3(256), 0, s, frames, pairs, B, P->col[0],
                          kcp(0));
-    else
+      KF_TRY(check_launch("im2col_pairs"));
+    } else if (cols) {
       hipLaunchKernelGGL(im2col_nhwc, dim3(blocks_for(M * c.kc)), dim3(256), 0, s, (const float*)P->Y[l - 1], l, B,
                          P->col[l]);
-    KF_TRY(check_launch("im2col"));
+      KF_TRY(check_launch("im2col_nhwc"));
+    }
     // Y[m][co] = sum_k col[m][k] W[co][k]; the layer-1 padding columns are zero in col.
     // Y = ReLU(col W^T + b): bias and ReLU in the GEMM epilogue
     Epilogue ep;
     ep.bias = (const float*)P->params + L.b[l];
     ep.relu = 1;
+    if (!cols && l + 1 < kConvs) ep.Cb = P->col[l + 1];  // BF16 NHWC copy, the next layer's input
+    const bool implicit = !cols && l > 0;
     KF_TRY(gemm(s, P->col[l], kcp(l), 1, P->wbf + L.w[l], 1, c.kc, P->Y[l], c.cout, (int)M, c.cout, c.kc, false,
-                nullptr, 0, ep));
+                nullptr, 0, ep, implicit ? &c : nullptr, implicit ? P->col[l] : nullptr));
   }
   hipLaunchKernelGGL(pool_concat, dim3(blocks_for((long)B * kFc1In)), dim3(256), 0, s, (const float*)P->Y[3], prev, B,
                      P->X5, P->X5b);
@@ -697,7 +757,7 @@ extern "C" kf_status kf_prior_forward(kf_prior* P, const float* frames, const in
                                       int32_t B, float* d_out, kf_stream s) {
   KF_TRY(check_batch(P, frames, pairs, prev, B));
   KF_CHECK_ARG(d_out != nullptr, "out is NULL");
-  return forward(P, frames, pairs, prev, B, S(s), d_out);
+  return forward(P, frames, pairs, prev, B, S(s), d_out, false);
 }
 
 extern "C" kf_status kf_prior_gradients(kf_prior* P, const float* frames, const int32_t* pairs, const float* prev,

@@ -34,6 +34,13 @@ def test_forward_and_gradients_match_oracle(gpu, kf):
     ref, _ = po.forward(params.astype(np.float64), frames, pairs, prev, bf=True)
     scale = np.abs(ref).max()
     assert np.max(np.abs(out - ref)) < 2e-3 * scale, (out, ref)
+    # Inference reads layers 2-4 straight from the activations; the training forward
+    # materialises im2col.  Same outputs: with the inference output as the target,
+    # the training forward's loss is exactly zero.
+    dl0 = kf.DeviceBuffer(4)
+    prior.gradients(df, dp, dv, dout, B, kf.DeviceBuffer(prior.nparams * 4), dl0)
+    kf.synchronize()
+    assert float(dl0.download(np.float32)[0]) == 0.0
     print("prior gradients B=4", flush=True)
     dg = kf.DeviceBuffer(prior.nparams * 4)
     dl = kf.DeviceBuffer(4)
