@@ -382,8 +382,17 @@ kf_status kf_pipeline_create(const kf_pipeline_desc* d, kf_stream s, kf_pipeline
 kf_status kf_pipeline_destroy(kf_pipeline* p);
 /* Clears the volume, sets frame 0's pose (host T_wc) and the frame counter. */
 kf_status kf_pipeline_reset(kf_pipeline* p, const double T0_wc[16]);
-/* Processes one uint16 depth frame already in device memory. */
+/* Processes one uint16 depth frame already in device memory (asynchronous on the
+ * pipeline stream).  As in RunApp::display (Source/src/MainApp.cpp:774-970), a call
+ * first integrates and raycasts the previous frame, then tracks this one: the pose
+ * of this frame is available when the call's work completes, its integration into
+ * the volume follows at the start of the next call (kf_pipeline_flush, which the
+ * volume and map accessors call, does it at once).  With a prior attached, the
+ * previous frame's integration and raycast run concurrently with this frame's
+ * preprocessing and prior inference (a second stream of the pipeline's own). */
 kf_status kf_pipeline_process(kf_pipeline* p, const uint16_t* d_depth);
+/* Integrates and raycasts the last tracked frame now if that is still pending. */
+kf_status kf_pipeline_flush(kf_pipeline* p);
 /* Attaches a learned prior (not owned; NULL detaches) before the next tracked frame.
  * Each frame then: prior guess -> ICP with icp_prior iterations -> accept, or fall
  * back to plain ICP from the constant-velocity guess when that ICP failed, its
@@ -399,11 +408,14 @@ kf_status kf_pipeline_get_pose(kf_pipeline* p, double T_wc[16]);          /* blo
 kf_status kf_pipeline_get_trajectory(kf_pipeline* p, double* T_out, int32_t max_frames,
                                      int32_t* frames); /* blocking */
 kf_status kf_pipeline_get_stats(kf_pipeline* p, kf_frame_stats* stats);   /* blocking */
+/* Both flush first: the volume holds every processed frame, the model maps are the
+ * raycast at the last pose, the live maps those of the last frame. */
 kf_status kf_pipeline_get_volume(kf_pipeline* p, kf_volume** vol);
 kf_status kf_pipeline_get_maps(kf_pipeline* p, kf_frame_maps* live, kf_frame_maps* model);
 kf_status kf_pipeline_get_stream(kf_pipeline* p, kf_stream* s);
 /* Stage-timing mode (eager launches, hipEvents on the pipeline stream, voxel and
- * march-step counters).  Times (ms) of the last processed frame:
+ * march-step counters; frames are processed in order without deferral or overlap).
+ * Times (ms) of the last processed frame:
  * [preprocess, guess+icp, raylen tiles + brick classes, sweep kernel, raycast kernel,
  *  model pyramid + pose log]; the sweep and raycast entries are the kernels' own
  * start-to-end times, the others the time between events around the stage. */

@@ -1,8 +1,15 @@
 // kf_pipeline.cpp — per-frame driver: preprocess -> track -> integrate -> raycast.
 //
 // Frame order and buffer ownership follow RunApp::display / initCudaPointers
-// (Source/src/MainApp.cpp:774-970, 591-647), rotated into the standard
-// KinectFusion order.  Differences from the reference driver, by design:
+// (Source/src/MainApp.cpp:774-970, 591-647): the reference integrates and raycasts
+// frame k-1, then preprocesses frame k and tracks it.  Here the call for frame k
+// does the same; with a prior attached, the previous frame's integration and
+// raycast run on the pipeline stream while frame k's preprocessing and prior
+// inference run on a second stream, and tracking starts when both are done.  The tracked
+// pose is therefore available when the call returns, and frame k is integrated at
+// the start of the next call (or by kf_pipeline_flush, which the volume and map
+// accessors call).  The two live frames alternate between two sets of maps.
+// Differences from the reference driver, by design:
 //  * the initial guess is the constant-velocity prediction T1 (T0^-1 T1) instead of
 //    the previous pose (Source/src/ICP.cpp:94);
 //  * nothing returns to the host inside a frame: poses, stats and the ICP solve
@@ -43,14 +50,17 @@ struct kf_pipeline {
   hipStream_t stream = nullptr;
   bool own_stream = false;  // created here when the caller passes NULL (graphs cannot capture the null stream)
   kf_volume* vol = nullptr;
-  kf_frame_maps live = {}, model = {};
+  kf_frame_maps live2[2] = {}, model = {};  // live maps of the even / odd frames
+  hipStream_t side = nullptr;               // preprocessing and prior inference of the new frame
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  bool pending = false;                     // the last tracked frame is not integrated yet
   kf_tracker* tracker = nullptr;
   kf_pose* d_poses = nullptr;        // kSlots poses
   kf_icp_stats* d_stats = nullptr;   // 2: plain, prior
   uint16_t* d_depth = nullptr;       // staging input buffer
   int frame = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph[2] = {};          // frame graphs for even / odd frames (live maps alternate)
+  hipGraphExec_t exec[2] = {};
   bool stage_timing = false;
   hipEvent_t ev[KF_STAGES + 1] = {};
   hipEvent_t kev[4] = {};  // own start / end of the sweep kernel, then of the raycast kernel
@@ -208,79 +218,120 @@ __global__ void prior_post(PriorState* st, const kf_icp_stats* sp, const kf_icp_
   }
 }
 
-kf_status stage_prior_inputs(kf_pipeline* p, hipStream_t s) {
+// Live maps of frame f (the two sets alternate).
+kf_frame_maps& live_of(kf_pipeline* p, int f) { return p->live2[f & 1]; }
+
+kf_status stage_prior_inputs(kf_pipeline* p, const kf_frame_maps& live, hipStream_t s) {
   const int n = KF_PRIOR_WIDTH * KF_PRIOR_HEIGHT;
-  hipLaunchKernelGGL(prior_stage_inputs, dim3(div_up(n, 256)), dim3(256), 0, s,
-                     (const float*)p->live.level[2].depth, p->d_prior_frames, n);
+  hipLaunchKernelGGL(prior_stage_inputs, dim3(div_up(n, 256)), dim3(256), 0, s, (const float*)live.level[2].depth,
+                     p->d_prior_frames, n);
   return check_launch("prior_stage_inputs");
 }
 
-kf_status record_track_prior(kf_pipeline* p, hipStream_t s) {
+// Frame k's part that does not depend on the previous frame's model: preprocessing,
+// the constant-velocity guess and, with a prior, its inference.
+kf_status record_front(kf_pipeline* p, kf_frame_maps& live, hipStream_t s) {
   const kf_pipeline_desc& d = p->d;
-  KF_TRY(stage_prior_inputs(p, s));
+  KF_TRY(launch_preprocess(d.intr, d.pre, p->d_depth, live, s));
   KF_TRY(launch_pose_constant_velocity(&p->d_poses[kPrev2], &p->d_poses[kPrev1], &p->d_poses[kGuess], s));
+  if (!p->prior) return KF_OK;
+  KF_TRY(stage_prior_inputs(p, live, s));
   char* st = reinterpret_cast<char*>(p->d_prior);  // device addresses of PriorState members
-  KF_TRY(kf_prior_forward(p->prior, p->d_prior_frames, reinterpret_cast<int32_t*>(st + offsetof(PriorState, pairs)),
+  return kf_prior_forward(p->prior, p->d_prior_frames, reinterpret_cast<int32_t*>(st + offsetof(PriorState, pairs)),
                           reinterpret_cast<float*>(st + offsetof(PriorState, prev_twist)), 1,
-                          reinterpret_cast<float*>(st + offsetof(PriorState, out_twist)), s));
-  hipLaunchKernelGGL(prior_guess, dim3(1), dim3(64), 0, s, p->d_prior, p->d_poses);
-  KF_TRY(check_launch("prior_guess"));
-  KF_TRY(launch_icp_track(p->tracker_prior, p->live, p->model, &p->d_poses[kGuessPrior], &p->d_poses[kPriorOut],
-                          &p->d_stats[1], s));
-  hipLaunchKernelGGL(prior_decide, dim3(1), dim3(64), 0, s, p->d_prior, (const kf_icp_stats*)&p->d_stats[1],
-                     &p->d_stats[0], p->d_poses, d.fallback_residual_ratio, d.fallback_min_inlier_frac);
-  KF_TRY(check_launch("prior_decide"));
-  // the fallback: plain ICP from the constant-velocity guess, skipped (by the flag
-  // prior_decide sets) in most frames, hence one launch for the whole call
-  KF_TRY(launch_icp_track(p->tracker, p->live, p->model, &p->d_poses[kGuess], &p->d_poses[kCur], &p->d_stats[0],
-                          s, true));
-  hipLaunchKernelGGL(prior_post, dim3(1), dim3(64), 0, s, p->d_prior, (const kf_icp_stats*)&p->d_stats[1],
-                     (const kf_icp_stats*)&p->d_stats[0], (const kf_pose*)p->d_poses);
-  return check_launch("prior_post");
+                          reinterpret_cast<float*>(st + offsetof(PriorState, out_twist)), s);
 }
 
-// One tracked frame.  In stage-timing mode (eager only) events bracket each stage
-// and the sweep kernel alone, and the sweep / raycast counters are enabled.
-kf_status record_track_frame(kf_pipeline* p, bool timed) {
-  hipStream_t s = p->stream;
+// Tracking of frame k against the model (after record_front): prior guess and
+// warm-started ICP with the fallback, or plain ICP; then the pose log and history.
+kf_status record_track(kf_pipeline* p, kf_frame_maps& live, hipStream_t s) {
   const kf_pipeline_desc& d = p->d;
-  unsigned long long* upd = timed ? &p->d_counters[0] : nullptr;
-  unsigned long long* steps = timed ? &p->d_counters[1] : nullptr;
-  if (timed) {
-    KF_HIP(hipMemsetAsync(p->d_counters, 0, kCounters * sizeof(unsigned long long), s));
-    KF_HIP(hipEventRecord(p->ev[0], s));
-  }
-  KF_TRY(launch_preprocess(d.intr, d.pre, p->d_depth, p->live, s));
-  if (timed) KF_HIP(hipEventRecord(p->ev[1], s));
   if (p->prior) {
-    KF_TRY(record_track_prior(p, s));
+    hipLaunchKernelGGL(prior_guess, dim3(1), dim3(64), 0, s, p->d_prior, p->d_poses);
+    KF_TRY(check_launch("prior_guess"));
+    KF_TRY(launch_icp_track(p->tracker_prior, live, p->model, &p->d_poses[kGuessPrior], &p->d_poses[kPriorOut],
+                            &p->d_stats[1], s));
+    hipLaunchKernelGGL(prior_decide, dim3(1), dim3(64), 0, s, p->d_prior, (const kf_icp_stats*)&p->d_stats[1],
+                       &p->d_stats[0], p->d_poses, d.fallback_residual_ratio, d.fallback_min_inlier_frac);
+    KF_TRY(check_launch("prior_decide"));
+    // the fallback: plain ICP from the constant-velocity guess, skipped (by the flag
+    // prior_decide sets) in most frames, hence one launch for the whole call
+    KF_TRY(launch_icp_track(p->tracker, live, p->model, &p->d_poses[kGuess], &p->d_poses[kCur], &p->d_stats[0], s,
+                            true));
+    hipLaunchKernelGGL(prior_post, dim3(1), dim3(64), 0, s, p->d_prior, (const kf_icp_stats*)&p->d_stats[1],
+                       (const kf_icp_stats*)&p->d_stats[0], (const kf_pose*)p->d_poses);
+    KF_TRY(check_launch("prior_post"));
   } else {
-    KF_TRY(launch_pose_constant_velocity(&p->d_poses[kPrev2], &p->d_poses[kPrev1], &p->d_poses[kGuess], s));
-    KF_TRY(launch_icp_track(p->tracker, p->live, p->model, &p->d_poses[kGuess], &p->d_poses[kCur],
-                            &p->d_stats[0], s));
+    KF_TRY(launch_icp_track(p->tracker, live, p->model, &p->d_poses[kGuess], &p->d_poses[kCur], &p->d_stats[0], s));
   }
-  if (timed) KF_HIP(hipEventRecord(p->ev[2], s));
-  KF_TRY(launch_integrate(p->vol, d.intr, d.integ, p->live.raylen, &p->d_poses[kCur], upd, nullptr, s,
-                          timed ? p->ev[3] : nullptr, timed ? p->kev[0] : nullptr, timed ? p->kev[1] : nullptr));
-  if (timed) KF_HIP(hipEventRecord(p->ev[4], s));
-  KF_TRY(launch_raycast(p->vol, d.intr, d.ray, &p->d_poses[kCur], p->model, steps, s, timed ? p->ev[5] : nullptr,
-                        timed ? &p->d_counters[2] : nullptr, timed ? p->kev[2] : nullptr,
-                        timed ? p->kev[3] : nullptr));
-  hipLaunchKernelGGL(pose_shift, dim3(1), dim3(64), 0, s, p->d_poses, p->d_traj, p->d_traj_count);
-  KF_TRY(check_launch("pose_shift"));
-  if (timed) KF_HIP(hipEventRecord(p->ev[6], s));
   return KF_OK;
 }
 
-kf_status bootstrap_frame(kf_pipeline* p) {
-  hipStream_t s = p->stream;
-  const kf_pipeline_desc& d = p->d;
-  KF_TRY(launch_preprocess(d.intr, d.pre, p->d_depth, p->live, s));
-  if (p->prior) KF_TRY(stage_prior_inputs(p, s));
-  KF_TRY(launch_integrate(p->vol, d.intr, d.integ, p->live.raylen, &p->d_poses[kCur], nullptr, nullptr, s));
-  KF_TRY(launch_raycast(p->vol, d.intr, d.ray, &p->d_poses[kCur], p->model, nullptr, s));
+kf_status record_pose_shift(kf_pipeline* p, hipStream_t s) {
   hipLaunchKernelGGL(pose_shift, dim3(1), dim3(64), 0, s, p->d_poses, p->d_traj, p->d_traj_count);
   return check_launch("pose_shift");
+}
+
+// Integration and raycast of the previous tracked frame (its pose is kPrev1 once
+// pose_shift has run).
+kf_status record_finish(kf_pipeline* p, const kf_frame_maps& live, hipStream_t s) {
+  const kf_pipeline_desc& d = p->d;
+  KF_TRY(launch_integrate(p->vol, d.intr, d.integ, live.raylen, &p->d_poses[kPrev1], nullptr, nullptr, s));
+  return launch_raycast(p->vol, d.intr, d.ray, &p->d_poses[kPrev1], p->model, nullptr, s);
+}
+
+// The call for frame k (k >= 1): [previous frame's integrate + raycast on the
+// pipeline stream] in parallel with [frame k's front part on the side stream], then
+// tracking and the pose log on the pipeline stream.  Without a prior the front
+// part is the preprocessing alone, which contends with the sweep and raycast for
+// the same units (measured slower in parallel), so then everything runs in order.
+kf_status record_call(kf_pipeline* p, int k, bool finish) {
+  hipStream_t s = p->stream;
+  if (!p->prior) {
+    if (finish) KF_TRY(record_finish(p, live_of(p, k - 1), s));
+    KF_TRY(record_front(p, live_of(p, k), s));
+    KF_TRY(record_track(p, live_of(p, k), s));
+    return record_pose_shift(p, s);
+  }
+  KF_HIP(hipEventRecord(p->fork_ev, s));
+  KF_HIP(hipStreamWaitEvent(p->side, p->fork_ev, 0));
+  if (finish) KF_TRY(record_finish(p, live_of(p, k - 1), s));
+  KF_TRY(record_front(p, live_of(p, k), p->side));
+  KF_HIP(hipEventRecord(p->join_ev, p->side));
+  KF_HIP(hipStreamWaitEvent(s, p->join_ev, 0));
+  KF_TRY(record_track(p, live_of(p, k), s));
+  return record_pose_shift(p, s);
+}
+
+// One frame in stage-timing mode (eager, one stream, nothing deferred): events
+// bracket each stage and the sweep / raycast kernels alone, and the sweep /
+// raycast counters are enabled.
+kf_status record_timed_frame(kf_pipeline* p, int k) {
+  hipStream_t s = p->stream;
+  const kf_pipeline_desc& d = p->d;
+  kf_frame_maps& live = live_of(p, k);
+  KF_HIP(hipMemsetAsync(p->d_counters, 0, kCounters * sizeof(unsigned long long), s));
+  KF_HIP(hipEventRecord(p->ev[0], s));
+  KF_TRY(launch_preprocess(d.intr, d.pre, p->d_depth, live, s));
+  KF_HIP(hipEventRecord(p->ev[1], s));
+  KF_TRY(launch_pose_constant_velocity(&p->d_poses[kPrev2], &p->d_poses[kPrev1], &p->d_poses[kGuess], s));
+  if (p->prior) {
+    KF_TRY(stage_prior_inputs(p, live, s));
+    char* st = reinterpret_cast<char*>(p->d_prior);
+    KF_TRY(kf_prior_forward(p->prior, p->d_prior_frames, reinterpret_cast<int32_t*>(st + offsetof(PriorState, pairs)),
+                            reinterpret_cast<float*>(st + offsetof(PriorState, prev_twist)), 1,
+                            reinterpret_cast<float*>(st + offsetof(PriorState, out_twist)), s));
+  }
+  KF_TRY(record_track(p, live, s));
+  KF_HIP(hipEventRecord(p->ev[2], s));
+  KF_TRY(launch_integrate(p->vol, d.intr, d.integ, live.raylen, &p->d_poses[kCur], &p->d_counters[0], nullptr, s,
+                          p->ev[3], p->kev[0], p->kev[1]));
+  KF_HIP(hipEventRecord(p->ev[4], s));
+  KF_TRY(launch_raycast(p->vol, d.intr, d.ray, &p->d_poses[kCur], p->model, &p->d_counters[1], s, p->ev[5],
+                        &p->d_counters[2], p->kev[2], p->kev[3]));
+  KF_TRY(record_pose_shift(p, s));
+  KF_HIP(hipEventRecord(p->ev[6], s));
+  return KF_OK;
 }
 
 kf_status reset_prior_state(kf_pipeline* p) {
@@ -294,10 +345,20 @@ kf_status reset_prior_state(kf_pipeline* p) {
 }
 
 void destroy_graph(kf_pipeline* p) {
-  if (p->exec) (void)hipGraphExecDestroy(p->exec);
-  if (p->graph) (void)hipGraphDestroy(p->graph);
-  p->exec = nullptr;
-  p->graph = nullptr;
+  for (int i = 0; i < 2; ++i) {
+    if (p->exec[i]) (void)hipGraphExecDestroy(p->exec[i]);
+    if (p->graph[i]) (void)hipGraphDestroy(p->graph[i]);
+    p->exec[i] = nullptr;
+    p->graph[i] = nullptr;
+  }
+}
+
+// Integrates and raycasts the last tracked frame if that is still pending.
+kf_status flush(kf_pipeline* p) {
+  if (!p->pending) return KF_OK;
+  KF_TRY(record_finish(p, live_of(p, p->frame - 1), p->stream));
+  p->pending = false;
+  return KF_OK;
 }
 
 }  // namespace
@@ -349,7 +410,11 @@ extern "C" kf_status kf_pipeline_destroy(kf_pipeline* p) {
   if (!p) return KF_OK;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   destroy_graph(p);
+  if (p->side) (void)hipStreamSynchronize(p->side);
   if (p->own_stream) (void)hipStreamDestroy(p->stream);
+  if (p->side) (void)hipStreamDestroy(p->side);
+  if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
+  if (p->join_ev) (void)hipEventDestroy(p->join_ev);
   for (hipEvent_t& e : p->ev)
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t& e : p->kev)
@@ -362,7 +427,8 @@ extern "C" kf_status kf_pipeline_destroy(kf_pipeline* p) {
   kf_tracker_destroy(p->tracker_prior);
   (void)hipFree(p->d_prior);
   (void)hipFree(p->d_prior_frames);
-  if (p->live.level[0].vx) kf_maps_free(&p->live);
+  for (kf_frame_maps& m : p->live2)
+    if (m.level[0].vx) kf_maps_free(&m);
   if (p->model.level[0].vx) kf_maps_free(&p->model);
   (void)hipFree(p->d_poses);
   (void)hipFree(p->d_stats);
@@ -389,7 +455,14 @@ extern "C" kf_status kf_pipeline_create(const kf_pipeline_desc* d, kf_stream s, 
   }
   st = kf_volume_create(&d->volume, &p->vol);
   if (st == KF_OK) st = kf_tracker_create(&d->intr, &d->icp, &p->tracker);
-  if (st == KF_OK) st = kf_maps_alloc(&d->intr, levels, 1, &p->live);
+  if (st == KF_OK) st = kf_maps_alloc(&d->intr, levels, 1, &p->live2[0]);
+  if (st == KF_OK) st = kf_maps_alloc(&d->intr, levels, 1, &p->live2[1]);
+  if (st == KF_OK) {
+    hipError_t e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming);
+    if (e != hipSuccess) st = hip_fail(e, "pipeline streams");
+  }
   if (st == KF_OK) st = kf_maps_alloc(&d->intr, levels, 0, &p->model);
   if (st == KF_OK) {
     hipError_t e = hipMalloc(&p->d_poses, kSlots * sizeof(kf_pose));
@@ -424,51 +497,65 @@ extern "C" kf_status kf_pipeline_reset(kf_pipeline* p, const double T0[16]) {
   if (p->d_prior) KF_TRY(reset_prior_state(p));
   KF_HIP(hipStreamSynchronize(p->stream));
   p->frame = 0;
+  p->pending = false;
   return KF_OK;
 }
 
 extern "C" kf_status kf_pipeline_process(kf_pipeline* p, const uint16_t* d_depth) {
   KF_CHECK_ARG(p && d_depth, "NULL argument");
   hipStream_t s = p->stream;
+  const int k = p->frame;
   const size_t bytes = (size_t)p->d.intr.width * p->d.intr.height * sizeof(uint16_t);
   KF_HIP(hipMemcpyAsync(p->d_depth, d_depth, bytes, hipMemcpyDeviceToDevice, s));
   // Voxels written outside the library since the last frame: rebuild the brick map
   // here, eagerly, so that it is never recorded into the frame graph.
   KF_TRY(ensure_occupancy(p->vol, s));
-  if (p->frame == 0) {
-    KF_TRY(bootstrap_frame(p));
-  } else if (p->stage_timing || !p->d.use_graph) {
-    KF_TRY(record_track_frame(p, p->stage_timing));
-    if (p->stage_timing) {
-      KF_HIP(hipEventSynchronize(p->ev[KF_STAGES]));
-      for (int i = 0; i < KF_STAGES; ++i)
-        KF_HIP(hipEventElapsedTime(&p->stage_ms[i], p->ev[i], p->ev[i + 1]));
-      // the sweep and raycast stages are their kernels' own start-to-end times
-      KF_HIP(hipEventElapsedTime(&p->stage_ms[3], p->kev[0], p->kev[1]));
-      KF_HIP(hipEventElapsedTime(&p->stage_ms[4], p->kev[2], p->kev[3]));
-    }
+  if (k == 0) {  // first frame: preprocess and log the initial pose; its integration follows
+    kf_frame_maps& live = live_of(p, 0);
+    KF_TRY(launch_preprocess(p->d.intr, p->d.pre, p->d_depth, live, s));
+    if (p->prior) KF_TRY(stage_prior_inputs(p, live, s));
+    KF_TRY(record_pose_shift(p, s));
+    p->pending = true;
+  } else if (p->stage_timing) {
+    KF_TRY(flush(p));
+    KF_TRY(record_timed_frame(p, k));
+    KF_HIP(hipEventSynchronize(p->ev[KF_STAGES]));
+    for (int i = 0; i < KF_STAGES; ++i) KF_HIP(hipEventElapsedTime(&p->stage_ms[i], p->ev[i], p->ev[i + 1]));
+    // the sweep and raycast stages are their kernels' own start-to-end times
+    KF_HIP(hipEventElapsedTime(&p->stage_ms[3], p->kev[0], p->kev[1]));
+    KF_HIP(hipEventElapsedTime(&p->stage_ms[4], p->kev[2], p->kev[3]));
+  } else if (!p->d.use_graph || !p->pending) {
+    KF_TRY(record_call(p, k, p->pending));
+    p->pending = true;
   } else {
-    if (!p->exec) {
+    const int g = k & 1;
+    if (!p->exec[g]) {
       KF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      kf_status st = record_track_frame(p, false);
-      hipGraph_t g = nullptr;
-      hipError_t e = hipStreamEndCapture(s, &g);
+      kf_status st = record_call(p, k, true);
+      hipGraph_t gr = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &gr);
       if (st != KF_OK) {
-        if (g) (void)hipGraphDestroy(g);
+        if (gr) (void)hipGraphDestroy(gr);
         return st;
       }
       if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-      p->graph = g;
-      e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+      p->graph[g] = gr;
+      e = hipGraphInstantiate(&p->exec[g], gr, nullptr, nullptr, 0);
       if (e != hipSuccess) {
         destroy_graph(p);
         return hip_fail(e, "hipGraphInstantiate");
       }
     }
-    KF_HIP(hipGraphLaunch(p->exec, s));
+    KF_HIP(hipGraphLaunch(p->exec[g], s));
+    p->pending = true;
   }
   ++p->frame;
   return KF_OK;
+}
+
+extern "C" kf_status kf_pipeline_flush(kf_pipeline* p) {
+  KF_CHECK_ARG(p != nullptr, "pipeline is NULL");
+  return flush(p);
 }
 
 extern "C" kf_status kf_pipeline_get_pose(kf_pipeline* p, double T[16]) {
@@ -545,13 +632,15 @@ extern "C" kf_status kf_pipeline_get_stats(kf_pipeline* p, kf_frame_stats* st) {
 
 extern "C" kf_status kf_pipeline_get_volume(kf_pipeline* p, kf_volume** v) {
   KF_CHECK_ARG(p && v, "NULL argument");
+  KF_TRY(flush(p));
   *v = p->vol;
   return KF_OK;
 }
 
 extern "C" kf_status kf_pipeline_get_maps(kf_pipeline* p, kf_frame_maps* live, kf_frame_maps* model) {
   KF_CHECK_ARG(p, "NULL argument");
-  if (live) *live = p->live;
+  KF_TRY(flush(p));
+  if (live) *live = live_of(p, p->frame > 0 ? p->frame - 1 : 0);
   if (model) *model = p->model;
   return KF_OK;
 }

@@ -182,6 +182,7 @@ _PROTOS = {
     "kf_pipeline_get_stats": (C.c_int, [_VP, C.POINTER(FrameStats)]),
     "kf_pipeline_get_trajectory": (C.c_int, [_VP, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32)]),
     "kf_pipeline_get_volume": (C.c_int, [_VP, C.POINTER(_VP)]),
+    "kf_pipeline_flush": (C.c_int, [_VP]),
     "kf_pipeline_get_maps": (C.c_int, [_VP, C.POINTER(FrameMaps), C.POINTER(FrameMaps)]),
     "kf_pipeline_get_stream": (C.c_int, [_VP, C.POINTER(_VP)]),
     "kf_pipeline_set_stage_timing": (C.c_int, [_VP, C.c_int32]),
@@ -745,6 +746,10 @@ class Pipeline:
         s = FrameStats()
         check(lib().kf_pipeline_get_stats(self.handle, C.byref(s)), "get_stats")
         return s
+
+    def flush(self) -> None:
+        """Integrates and raycasts the last tracked frame now (kf_pipeline_flush)."""
+        check(lib().kf_pipeline_flush(self.handle), "flush")
 
     def volume(self) -> Volume:
         h = _VP()

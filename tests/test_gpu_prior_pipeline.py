@@ -43,6 +43,32 @@ def test_forced_fallback_reproduces_plain_pipeline(gpu, kf, synth, small):
     assert np.array_equal(V_plain, V_fb)
 
 
+def test_overlapped_frames_match_in_order_frames(gpu, kf, synth, small):
+    """With a prior, a call integrates the previous frame while the new frame is
+    preprocessed and run through the prior on a second stream (graph replay); in
+    stage-timing mode every frame runs in order on one stream.  Same poses, same
+    volume after the flush."""
+    print("prior pipeline: overlapped vs in-order frames", flush=True)
+    prior = kf.Prior(max_batch=1)
+    prior.init(5)
+    d = synth.pipeline_desc("small")
+    F = 12
+    dd = kf.upload(small.depth[:F])
+    out = []
+    for timed in (False, True):
+        p = kf.Pipeline(d)
+        p.attach_prior(prior)
+        p.set_stage_timing(timed)
+        p.reset(small.poses[0])
+        for i in range(F):
+            p.process(dd.ptr + i * small.depth[0].nbytes)
+        out.append((p.trajectory(), p.volume().download_raw()))
+        p.close()
+    assert out[0][0].shape[0] == F
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_warm_started_tracking_without_fallback(gpu, kf, synth, small):
     print("prior pipeline: never fall back", flush=True)
     prior = kf.Prior(max_batch=1)
