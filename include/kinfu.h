@@ -160,10 +160,12 @@ kf_status kf_volume_mark_modified(kf_volume* vol);
  * every step).  Rebuilds a stale map on stream s first. */
 kf_status kf_volume_occupancy(const kf_volume* vol, const uint64_t** d_words, int32_t* bricks_per_axis,
                               kf_stream s);
-/* Its coarse level: super-bricks of 4x4x4 bricks (32^3 voxels), word sx*nsb + sy,
- * bit sz set iff one of the super-brick's bricks is set (nsb = ceil(n / 32)). */
-kf_status kf_volume_occupancy_coarse(const kf_volume* vol, const uint32_t** d_words, int32_t* per_axis,
-                                     kf_stream s);
+/* Brick distances behind the raycaster's free-space skip: d_dist[(bx*nb + by)*64 + bz]
+ * = min(Chebyshev distance in bricks to the nearest set brick of the occupancy map,
+ * 16), computed from the current map (enqueued on s).  No counterpart in the
+ * reference (it marches every step); NULL / 0 when the volume has no map. */
+kf_status kf_volume_brick_distance(const kf_volume* vol, const uint8_t** d_dist, int32_t* bricks_per_axis,
+                                   kf_stream s);
 /* Occupancy statistics over every voxel (reads the whole volume once):
  * d_out (device, 4 doubles) = {observed voxels (w > 0), sum of w, sum of f over
  * observed voxels, voxels with f < 0 and w > 0}. */

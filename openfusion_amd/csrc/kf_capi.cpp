@@ -274,16 +274,13 @@ extern "C" kf_status kf_volume_create(const kf_volume_desc* d, kf_volume** out) 
   }
   if (d->n / 8 <= kf::kOccMaxBricks) {
     v->occ_nb = d->n / 8;
-    v->occ_nsb = (v->occ_nb + 3) / 4;
-    // brick words, then the super-brick words, rounded up to whole 8-byte words (the
-    // clearing kernel writes 8-byte words over both)
-    e = hipMalloc(&v->d_occ, (size_t)v->occ_nb * v->occ_nb * sizeof(unsigned long long) +
-                                 (((size_t)v->occ_nsb * v->occ_nsb + 1) / 2) * sizeof(unsigned long long));
-    if (e == hipSuccess) v->d_occ2 = reinterpret_cast<unsigned*>(v->d_occ + (size_t)v->occ_nb * v->occ_nb);
+    e = hipMalloc(&v->d_occ, (size_t)v->occ_nb * v->occ_nb * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&v->d_bdist, (size_t)v->occ_nb * v->occ_nb * kf::kBrickDistRow);
     if (e != hipSuccess) {
       (void)hipFree(v->d_voxels);
       (void)hipFree(v->d_bricks);
       (void)hipFree(v->d_rtiles);
+      (void)hipFree(v->d_occ);
       delete v;
       return hip_fail(e, "hipMalloc(occupancy)");
     }
@@ -298,6 +295,7 @@ extern "C" kf_status kf_volume_create(const kf_volume_desc* d, kf_volume** out) 
     (void)hipFree(v->d_bricks);
     (void)hipFree(v->d_rtiles);
     (void)hipFree(v->d_occ);
+    (void)hipFree(v->d_bdist);
     delete v;
     return st;
   }
@@ -311,6 +309,7 @@ extern "C" kf_status kf_volume_destroy(kf_volume* v) {
   (void)hipFree(v->d_bricks);
   (void)hipFree(v->d_rtiles);
   (void)hipFree(v->d_occ);
+  (void)hipFree(v->d_bdist);
   delete v;
   return KF_OK;
 }
@@ -347,12 +346,14 @@ extern "C" kf_status kf_volume_mark_modified(kf_volume* v) {
   return KF_OK;
 }
 
-extern "C" kf_status kf_volume_occupancy_coarse(const kf_volume* v, const uint32_t** d_words, int32_t* per_axis,
-                                                kf_stream s) {
-  KF_CHECK_ARG(v && d_words && per_axis, "NULL argument");
+
+extern "C" kf_status kf_volume_brick_distance(const kf_volume* v, const uint8_t** d_dist, int32_t* bricks_per_axis,
+                                              kf_stream s) {
+  KF_CHECK_ARG(v && d_dist && bricks_per_axis, "NULL argument");
   KF_TRY(kf::ensure_occupancy(v, S(s)));
-  *d_words = reinterpret_cast<const uint32_t*>(v->d_occ2);
-  *per_axis = v->occ_nsb;
+  KF_TRY(kf::launch_brick_distance(v, S(s)));
+  *d_dist = v->d_bdist;
+  *bricks_per_axis = v->d_occ ? v->occ_nb : 0;
   return KF_OK;
 }
 

@@ -135,23 +135,11 @@ __device__ __forceinline__ unsigned long long occ_zbits(int z) {
 // wave, uniform control flow).  Up to four brick columns receive the bits (the
 // column's own and, on a brick boundary, the one below in x and/or y); a word is
 // only written when bits are missing.
-// Brick occupancy map plus its coarse level: super-bricks of 4x4x4 bricks (32^3
-// voxels), one 32-bit word per (sx, sy) with bit sz, set iff one of its bricks is.
+// Brick occupancy map: one 64-bit word per brick column (bx, by), bit bz.
 struct OccMap {
   unsigned long long* words;  // nb * nb, bit bz
-  unsigned* super;            // nsb * nsb, bit sz
-  int nb, nsb;
+  int nb;
 };
-
-// Bit i of the result = OR of bits 4i..4i+3 of m.
-__device__ __forceinline__ unsigned occ_compress4(unsigned long long m) {
-  m |= m >> 1;
-  m |= m >> 2;
-  unsigned r = 0;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) r |= (unsigned)((m >> (4 * i)) & 1ull) << i;
-  return r;
-}
 
 __device__ __forceinline__ void occ_mark_column(const OccMap& occ, int ix, int iy, unsigned long long bits,
                                                 int lane) {
@@ -166,13 +154,7 @@ __device__ __forceinline__ void occ_mark_column(const OccMap& occ, int ix, int i
     if (bx >= 0 && by >= 0) {
       unsigned long long* w = occ.words + (bx * occ.nb + by);
       const unsigned long long cur = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if ((cur & zm) != zm) {
-        __hip_atomic_fetch_or(w, zm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned sm = occ_compress4(zm);
-        unsigned* sw = occ.super + ((bx >> 2) * occ.nsb + (by >> 2));
-        const unsigned scur = __hip_atomic_load(sw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((scur & sm) != sm) __hip_atomic_fetch_or(sw, sm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      if ((cur & zm) != zm) __hip_atomic_fetch_or(w, zm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }

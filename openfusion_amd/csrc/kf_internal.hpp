@@ -58,9 +58,11 @@ struct kf_volume {
   // of a trilinear sample whose base voxel lies in the brick).  The sweep sets
   // bits as it writes f != 1; bits are never cleared except by a reset or a rebuild.
   unsigned long long* d_occ = nullptr;
-  unsigned* d_occ2 = nullptr;    // super-brick level (32^3 voxels): occ_nsb^2 words, bit sz
+  // Chebyshev distance, in bricks, from each brick to the nearest set brick bit,
+  // capped at kBrickDistCap (one byte per brick, (bx * nb + by) * kBrickDistRow + bz);
+  // recomputed before every raycast.
+  uint8_t* d_bdist = nullptr;
   int occ_nb = 0;                // bricks per axis (0: no map)
-  int occ_nsb = 0;               // super-bricks per axis
   mutable bool occ_stale = false;  // voxels may have been written outside the library
 };
 
@@ -102,7 +104,11 @@ kf_status launch_integrate(kf_volume* v, const kf_intrinsics& k0, const kf_integ
 kf_status launch_volume_stats(const kf_volume* v, double* d_out, hipStream_t s);
 // Recomputes the occupancy map from the voxels when it is stale (no-op otherwise).
 kf_status ensure_occupancy(const kf_volume* v, hipStream_t s);
+// Brick distances from the current occupancy map (no-op without a map).
+constexpr int kBrickDistCap = 16;
+kf_status launch_brick_distance(const kf_volume* v, hipStream_t s);
 constexpr int kOccMaxBricks = 64;  // bricks per axis held in one 64-bit word
+constexpr int kBrickDistRow = kOccMaxBricks;  // bytes per column of brick distances
 constexpr int kMaxImage = 8192;    // largest image side accepted by the API
 // raylen tiles: 8x8 then 32x32 pixels, (min, max) per tile
 constexpr size_t kRayTiles8 = (size_t)(kMaxImage / 8) * (kMaxImage / 8);

@@ -31,8 +31,8 @@ struct RayArgs {
   unsigned long long* steps;
   unsigned long long* diag;  // optional [samples that loaded corners, batches, skipped steps, 64 x wave max batches]
   const unsigned long long* occ;  // brick occupancy map or NULL
-  const unsigned* occ2;           // its super-brick level (32^3 voxels)
-  int occ_nb, occ_nsb;
+  const uint8_t* bdist;           // brick distances to the nearest set brick (with occ)
+  int occ_nb;
   int n;
   float h, inv_h, ox, oy, oz;
   float fx, fy, cx, cy;
@@ -105,14 +105,15 @@ __device__ __forceinline__ void load_corners(const RayArgs& a, voff_t<OFF32> o, 
 // floor(g), weights g - floor(g).  When the brick of the base voxel is clear in
 // the occupancy map, all eight corners hold f == 1 and the interpolation is
 // exactly 1 (a + (b - a) t with a = b = 1), so the voxel loads are skipped: the
-// value is the same.  Three phases, so that a batch costs about one map latency
+// value is the same.  dclear (optional): the brick distance of each sample whose
+// brick is clear (>= 1), 0 otherwise.  Three phases, so that a batch costs about one map latency
 // plus one voxel latency instead of K of each: the map words of all samples, then
 // the corners of all samples (one wave-uniform branch; lanes whose sample needs
 // no corners read voxel 0 instead, which stays cached), then the interpolations.
 // Returns the number of samples that needed corners.
 template <int FMT, bool OFF32, int K>
 __device__ __forceinline__ int sample_points(const RayArgs& a, const f3* pts, bool* ok, float* fk,
-                                             bool* clear = nullptr) {
+                                             int* dclear = nullptr) {
   const float lim = (float)(a.n - 1);
   float gx[K], gy[K], gz[K];
   int ix[K], iy[K], iz[K];
@@ -132,24 +133,30 @@ __device__ __forceinline__ int sample_points(const RayArgs& a, const f3* pts, bo
   if (a.occ) {
     // the 32-bit half of the brick column word that holds bit iz / 8
     const char* occ = reinterpret_cast<const char*>(a.occ);
-    unsigned w[K];
+    unsigned w[K], dj[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       const unsigned col = opaque_u32(__umul24((unsigned)(ix[j] >> 3), (unsigned)a.occ_nb)) + (unsigned)(iy[j] >> 3);
       const unsigned off = opaque_u32(((col << 1) | (unsigned)(iz[j] >> 8)) * 4u);
       w[j] = *reinterpret_cast<const unsigned*>(occ + off);
+      if (dclear) dj[j] = a.bdist[opaque_u32(col * (unsigned)kBrickDistRow + (unsigned)(iz[j] >> 3))];
     }
 #pragma unroll
-    for (int j = 0; j < K; ++j) need[j] = ok[j] && ((w[j] >> ((iz[j] >> 3) & 31)) & 1u);
+    for (int j = 0; j < K; ++j) {
+      need[j] = ok[j] && ((w[j] >> ((iz[j] >> 3) & 31)) & 1u);
+      if (dclear) dclear[j] = ok[j] && !need[j] ? (int)dj[j] : 0;
+    }
   } else {
 #pragma unroll
-    for (int j = 0; j < K; ++j) need[j] = ok[j];
+    for (int j = 0; j < K; ++j) {
+      need[j] = ok[j];
+      if (dclear) dclear[j] = 0;
+    }
   }
   int loads = 0, any = 0;
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     fk[j] = 1.0f;
-    if (clear) clear[j] = ok[j] && !need[j];
     loads += need[j] ? 1 : 0;
     any |= need[j] ? 1 : 0;
   }
@@ -186,11 +193,11 @@ __device__ __forceinline__ bool sample(const RayArgs& a, f3 p, float& out) {
 // K samples along the ray at parameters tk (positions o + t dir, as in sample()).
 template <int FMT, bool OFF32, int K>
 __device__ __forceinline__ int sample_batch(const RayArgs& a, f3 o, f3 dir, const float* tk, bool* ok,
-                                            float* fk, bool* clear) {
+                                            float* fk, int* dclear) {
   f3 pts[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) pts[j] = f3{o.x + tk[j] * dir.x, o.y + tk[j] * dir.y, o.z + tk[j] * dir.z};
-  return sample_points<FMT, OFF32, K>(a, pts, ok, fk, clear);
+  return sample_points<FMT, OFF32, K>(a, pts, ok, fk, dclear);
 }
 
 template <int FMT, bool OFF32>
@@ -201,10 +208,6 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
   const int xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
   const int logical = xcd * per + slot;
   if (logical >= nb) return;  // whole block
-  __shared__ unsigned s_super[256];  // super-brick words (occ_nsb <= 16)
-  const int nsw = a.occ2 ? a.occ_nsb * a.occ_nsb : 0;
-  if ((int)threadIdx.x < nsw) s_super[threadIdx.x] = a.occ2[threadIdx.x];
-  __syncthreads();
   const int by = logical / a.blocks_x, bx = logical - by * a.blocks_x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int x = bx * 16 + (wave & 1) * 8 + (lane & 7);
@@ -282,24 +285,23 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
     bool ok = sample<FMT, OFF32>(a, f3{o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z}, fp) && fp >= 0.0f;
     int s = 0;
     // Free-space skip.  From a far-mode position t (sample 1) whose base voxel lies
-    // in a clear box — its 32^3 super-brick when that is clear, else its brick when
-    // the brick is clear — advance k far steps at once: the k positions are the
-    // sequential ones (t += step, k additions), and the grid coordinate of a
-    // position is a monotone function of t on each axis, so if the last one is
-    // still in the box (checked with the exact sample() arithmetic) every
-    // intermediate one is too, and each of their samples is exactly 1.  The step
-    // count, the position and the value are those of the step-by-step march.
-    auto skip_free = [&](float& t_, int& s_) {
+    // in a brick at distance D >= 1 from the nearest set brick, every brick within
+    // D - 1 of it (a cube of 2D - 1 bricks) is clear, so every sample whose base
+    // voxel lies in that box is exactly 1.  Advance k far steps at once: the k
+    // positions are the sequential ones (t += step, k additions), and the grid
+    // coordinate of a position is a monotone function of t on each axis, so if the
+    // last one is still in the box (checked with the exact sample() arithmetic) every
+    // intermediate one is too.  The step count, the position and the value are those
+    // of the step-by-step march.
+    auto skip_free = [&](float& t_, int& s_, int D) {
       const float gx = ((o.x + t_ * dir.x) - a.ox) * a.inv_h - 0.5f;
       const float gy = ((o.y + t_ * dir.y) - a.oy) * a.inv_h - 0.5f;
       const float gz = ((o.z + t_ * dir.z) - a.oz) * a.inv_h - 0.5f;
-      const int ix = (int)gx, iy = (int)gy, iz = (int)gz;
-      const bool super_clear = !((s_super[(ix >> 5) * a.occ_nsb + (iy >> 5)] >> (iz >> 5)) & 1u);
-      const int sh = super_clear ? 5 : 3;  // box: super-brick (32) or brick (8)
+      const int bx = (int)gx >> 3, by = (int)gy >> 3, bz = (int)gz >> 3, r = D - 1;
       const float lim = (float)(a.n - 1);
-      const float lx = (float)((ix >> sh) << sh), hx = fminf((float)(((ix >> sh) + 1) << sh), lim);
-      const float ly = (float)((iy >> sh) << sh), hy = fminf((float)(((iy >> sh) + 1) << sh), lim);
-      const float lz = (float)((iz >> sh) << sh), hz = fminf((float)(((iz >> sh) + 1) << sh), lim);
+      const float lx = (float)(8 * max(bx - r, 0)), hx = fminf((float)(8 * (bx + r + 1)), lim);
+      const float ly = (float)(8 * max(by - r, 0)), hy = fminf((float)(8 * (by + r + 1)), lim);
+      const float lz = (float)(8 * max(bz - r, 0)), hz = fminf((float)(8 * (bz + r + 1)), lim);
       // Approximate exit parameter of the box (grid value G sits at world
       // (G + 0.5) h + origin), with reciprocals: only the estimate of k depends on
       // it, and the skip is verified with the exact arithmetic below.
@@ -325,13 +327,14 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
         n_skipped += k;
       }
     };
-    bool last_clear = false;  // the sample at t has its base voxel in a clear brick
+    int last_d = 0;  // brick distance of the sample at t when its brick is clear, else 0
     while (ok && s < a.max_steps) {
-      if (nsw && last_clear && fp >= 1.0f) skip_free(t, s);
+      if (last_d > 0 && fp >= 1.0f) skip_free(t, s, last_d);
       const bool far = fp >= 1.0f;
       const float st = far ? a.step_far_m : a.step_near_m;
       float tk[kSpec], fk[kSpec];
-      bool vk[kSpec], ck[kSpec];
+      bool vk[kSpec];
+      int ck[kSpec];
       float tt = t;
 #pragma unroll
       for (int j = 0; j < kSpec; ++j) {
@@ -359,7 +362,7 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
         }
         t = tk[j];
         fp = fk[j];
-        last_clear = ck[j];
+        last_d = ck[j];
         if ((fp >= 1.0f) != far) break;  // the next step has the other size
       }
       if (done) break;
@@ -453,10 +456,10 @@ kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_r
   a.steps = d_steps;
   a.diag = d_diag;
   KF_TRY(ensure_occupancy(v, s));
+  KF_TRY(launch_brick_distance(v, s));
   a.occ = v->d_occ;
-  a.occ2 = v->d_occ2;
+  a.bdist = v->d_bdist;
   a.occ_nb = v->occ_nb;
-  a.occ_nsb = v->occ_nsb;
   a.n = d.n;
   a.h = d.extent_m / (float)d.n;
   a.inv_h = (float)d.n / d.extent_m;

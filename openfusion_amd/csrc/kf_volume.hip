@@ -297,17 +297,12 @@ __device__ __forceinline__ bool voxel_finish(const SweepArgs& a, float rl, float
   return true;
 }
 
-// Marks brick (bx, by, bz) and its super-brick in the occupancy map (only when the
-// bits are missing).
+// Marks brick (bx, by, bz) in the occupancy map (only when the bit is missing).
 __device__ __forceinline__ void occ_mark_brick(const OccMap& occ, int bx, int by, int bz) {
   unsigned long long* w = occ.words + (bx * occ.nb + by);
   const unsigned long long bit = 1ull << bz;
   if (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit) return;
   __hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  unsigned* sw = occ.super + ((bx >> 2) * occ.nsb + (by >> 2));
-  const unsigned sbit = 1u << (bz >> 2);
-  if (!(__hip_atomic_load(sw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & sbit))
-    __hip_atomic_fetch_or(sw, sbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // The sweep over the listed bricks.  Lane (yy, zz) = (lane >> 3, lane & 7) holds
@@ -499,6 +494,121 @@ __global__ __launch_bounds__(256) void occupancy_rebuild(const void* vox, int n,
   }
 }
 
+// Brick distances, min(Chebyshev distance in bricks to the nearest set brick,
+// kBrickDistCap).  The distance of brick (x, y, z) is the number of radii r in
+// 0..cap-1 for which the cube of radius r around it holds no set brick; the cube's
+// set bricks are the bits of the OR over the (2r+1)^2 columns around (x, y) (bit bz
+// of column word (bx, by)), dilated by r along the word.  The square-window ORs come
+// from a sparse table in LDS: T_k(x, y) = OR over the 2^k x 2^k columns from (x, y),
+// built for k = 1..4 by doubling, and a window of side L is the OR of four T_k with
+// 2^k <= L < 2^(k+1).  One block per 8 x 8 tile of columns stages the tile plus a
+// halo of cap - 1 columns (out-of-volume columns empty) and builds the table; then
+// lane t of the first wavefront owns column (x0 + t % 8, y0 + t / 8), counts its 64
+// distances in five bit planes and writes them as one 64-byte row.
+constexpr int kDistTile = 8;
+constexpr int kDistHalo = kBrickDistCap - 1;
+constexpr int kDistSide = kDistTile + 2 * kDistHalo;
+constexpr int kDistLevels = 5;  // T_0 .. T_4: windows up to 2 * 15 + 1 = 31 < 32 columns
+constexpr int kDistThreads = 256;
+static_assert(2 * kDistHalo + 1 < (1 << kDistLevels), "window wider than the largest table level");
+
+// OR of w shifted by -r .. r bits (bits leaving the word are bricks outside the volume)
+__device__ __forceinline__ unsigned long long dilate_bits(unsigned long long w, int r) {
+  unsigned long long up = w, down = w;
+  int len = 1;
+  while (2 * len <= r + 1) {
+    up |= up << len;
+    down |= down >> len;
+    len *= 2;
+  }
+  const int rest = r + 1 - len;  // up = OR over shifts 0 .. len-1; extend to 0 .. r
+  up |= up << rest;
+  down |= down >> rest;
+  return up | down;
+}
+
+__global__ __launch_bounds__(kDistThreads) void brick_distance(OccMap occ, uint8_t* out) {
+  __shared__ unsigned long long t[kDistLevels][kDistSide * kDistSide];
+  constexpr int kCells = kDistSide * kDistSide;
+  constexpr int kPer = (kCells + kDistThreads - 1) / kDistThreads;  // cells per thread and level
+  const int nb = occ.nb, tid = threadIdx.x;
+  const int x0 = blockIdx.x * kDistTile - kDistHalo, y0 = blockIdx.y * kDistTile - kDistHalo;
+  {
+    // every load in flight at once: out-of-range cells read a clamped column and drop it
+    unsigned long long v[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int i = tid + j * kDistThreads;
+      const int x = x0 + i / kDistSide, y = y0 + i % kDistSide;
+      const bool in = i < kCells && x >= 0 && x < nb && y >= 0 && y < nb;
+      const int xc = min(max(x, 0), nb - 1), yc = min(max(y, 0), nb - 1);
+      const unsigned long long w = occ.words[xc * nb + yc];
+      v[j] = in ? w : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j)
+      if (tid + j * kDistThreads < kCells) t[0][tid + j * kDistThreads] = v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 1; k < kDistLevels; ++k) {
+    // T_k(a, b) for a, b + 2^k <= side from four T_(k-1) of half the side; all reads
+    // of a level before its writes (cells past the end recompute the last one)
+    const int h = 1 << (k - 1), n = kDistSide - (1 << k) + 1;
+    unsigned long long v[kPer];
+    int at[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int i = min(tid + j * kDistThreads, n * n - 1);
+      at[j] = (i / n) * kDistSide + i % n;
+      const unsigned long long* p = &t[k - 1][at[j]];
+      v[j] = p[0] | p[h] | p[h * kDistSide] | p[h * kDistSide + h];
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j)
+      if (tid + j * kDistThreads < n * n) t[k][at[j]] = v[j];
+    __syncthreads();
+  }
+  if (tid >= 64) return;  // one wavefront counts the tile's distances
+  const int lane = tid;
+  const int tx = lane % kDistTile, ty = lane / kDistTile;
+  const int ox = blockIdx.x * kDistTile + tx, oy = blockIdx.y * kDistTile + ty;
+  if (ox >= nb || oy >= nb) return;
+  const int lx = kDistHalo + tx, ly = kDistHalo + ty;
+  unsigned long long cnt[5] = {0ull, 0ull, 0ull, 0ull, 0ull};  // bit planes of the counts
+#pragma unroll
+  for (int r = 0; r < kBrickDistCap; ++r) {
+    const int k = 31 - __builtin_clz(2 * r + 1), side = 1 << k;
+    const int a = lx - r, b = lx + r + 1 - side, c = ly - r, d = ly + r + 1 - side;
+    const unsigned long long* T = t[k];
+    const unsigned long long w = T[a * kDistSide + c] | T[b * kDistSide + c] | T[a * kDistSide + d] |
+                                 T[b * kDistSide + d];
+    unsigned long long carry = ~dilate_bits(w, r);  // bricks with no set brick within r
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const unsigned long long nc = cnt[q] & carry;
+      cnt[q] ^= carry;
+      carry = nc;
+    }
+  }
+  // byte bz of the row = count of brick bz; 4 bricks per dword, bit j of a nibble
+  // spread to bit 8j by a multiply
+  uint32_t dw[16];
+#pragma unroll
+  for (int g = 0; g < 16; ++g) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const uint32_t nib = (uint32_t)(cnt[q] >> (4 * g)) & 0xFu;
+      v |= ((nib * 0x00204081u) & 0x01010101u) << q;
+    }
+    dw[g] = v;
+  }
+  uint4* row = reinterpret_cast<uint4*>(out + ((size_t)ox * nb + oy) * kBrickDistRow);
+#pragma unroll
+  for (int g = 0; g < 4; ++g) row[g] = make_uint4(dw[4 * g], dw[4 * g + 1], dw[4 * g + 2], dw[4 * g + 3]);
+}
+
 __global__ __launch_bounds__(256) void fill_u64(unsigned long long* p, int count, unsigned long long v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = v;
@@ -526,12 +636,12 @@ kf_status launch_volume_stats(const kf_volume* v, double* d_out, hipStream_t s) 
 }
 
 OccMap occ_map(const kf_volume* v) {
-  return OccMap{v->d_occ, v->d_occ2, v->occ_nb, v->occ_nsb};
+  return OccMap{v->d_occ, v->occ_nb};
 }
 
-// Both levels to zero (a fill kernel, so that graph capture stays valid).
+// The map to zero (a fill kernel, so that graph capture stays valid).
 kf_status clear_occupancy(const kf_volume* v, hipStream_t s) {
-  const int words = v->occ_nb * v->occ_nb + (v->occ_nsb * v->occ_nsb + 1) / 2;  // the whole allocation
+  const int words = v->occ_nb * v->occ_nb;
   hipLaunchKernelGGL(fill_u64, dim3(div_up(words, 256)), dim3(256), 0, s, v->d_occ, words, 0ull);
   return check_launch("fill_u64(occupancy)");
 }
@@ -553,6 +663,14 @@ kf_status launch_volume_reset(kf_volume* v, hipStream_t s) {
   if (v->d_occ) KF_TRY(clear_occupancy(v, s));  // every voxel is free space: all bricks clear
   v->occ_stale = false;
   return KF_OK;
+}
+
+kf_status launch_brick_distance(const kf_volume* v, hipStream_t s) {
+  if (!v->d_occ) return KF_OK;
+  const int nb = v->occ_nb;
+  const int tiles = div_up(nb, kDistTile);
+  hipLaunchKernelGGL(brick_distance, dim3(tiles, tiles), dim3(kDistThreads), 0, s, occ_map(v), v->d_bdist);
+  return check_launch("brick_distance");
 }
 
 kf_status ensure_occupancy(const kf_volume* v, hipStream_t s) {

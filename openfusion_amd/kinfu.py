@@ -144,8 +144,8 @@ _PROTOS = {
     "kf_volume_stats": (C.c_int, [_VP, _VP, _VP]),
     "kf_volume_mark_modified": (C.c_int, [_VP]),
     "kf_volume_data_const": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(C.c_size_t)]),
+    "kf_volume_brick_distance": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(C.c_int32), _VP]),
     "kf_volume_occupancy": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(C.c_int32), _VP]),
-    "kf_volume_occupancy_coarse": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(C.c_int32), _VP]),
     "kf_volume_export_tsdf": (C.c_int, [_VP, _VP, _VP]),
     "kf_volume_extract_mesh": (C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP]),
     "kf_depth_median": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, C.c_uint16, _VP]),
@@ -469,18 +469,18 @@ class Volume:
         bits = (words[:, :, None] >> np.arange(nb.value, dtype=np.uint64)[None, None, :]) & np.uint64(1)
         return bits.astype(bool)
 
-    def occupancy_coarse(self) -> np.ndarray:
-        """Super-brick level (32^3 voxels) as (nsb, nsb, nsb) bool [sx, sy, sz]."""
+    def brick_distance(self) -> np.ndarray:
+        """Brick distances (nb, nb, nb) uint8 [bx, by, bz]: min(Chebyshev distance in
+        bricks to the nearest set brick, 16); empty when the volume has no map."""
         p = _VP()
-        nsb = C.c_int32()
-        check(lib().kf_volume_occupancy_coarse(self.handle, C.byref(p), C.byref(nsb), None),
-              "kf_volume_occupancy_coarse")
+        nb = C.c_int32()
+        check(lib().kf_volume_brick_distance(self.handle, C.byref(p), C.byref(nb), None),
+              "kf_volume_brick_distance")
         synchronize()
-        if nsb.value == 0:
-            return np.zeros((0, 0, 0), bool)
-        words = download_ptr(p.value, np.uint32, nsb.value * nsb.value).reshape(nsb.value, nsb.value)
-        bits = (words[:, :, None] >> np.arange(nsb.value, dtype=np.uint32)[None, None, :]) & np.uint32(1)
-        return bits.astype(bool)
+        if nb.value == 0:
+            return np.zeros((0, 0, 0), np.uint8)
+        rows = download_ptr(p.value, np.uint8, nb.value * nb.value * 64).reshape(nb.value, nb.value, 64)
+        return rows[:, :, : nb.value].copy()
 
     def stats(self) -> np.ndarray:
         """[observed voxels, sum w, sum f (observed), observed voxels with f < 0]."""

@@ -4,6 +4,7 @@ exact map while the sweep maintains it, and invisible in raycast output, which t
 parity tests compare bit for bit with the oracle's full march."""
 import numpy as np
 import pytest
+from scipy import ndimage
 
 from tests.helpers import fill_volume, integ_params, vol_desc, voxel_centres
 
@@ -26,13 +27,12 @@ def exact_occupancy(f: np.ndarray) -> np.ndarray:
     return a
 
 
-def coarse_of(occ: np.ndarray) -> np.ndarray:
-    """Super-brick level: OR over groups of 4x4x4 bricks."""
-    nb = occ.shape[0]
-    nsb = (nb + 3) // 4
-    pad = np.zeros((4 * nsb,) * 3, bool)
-    pad[:nb, :nb, :nb] = occ
-    return pad.reshape(nsb, 4, nsb, 4, nsb, 4).any(axis=(1, 3, 5))
+def brick_distance_of(occ: np.ndarray, cap: int = 16) -> np.ndarray:
+    """min(Chebyshev distance in bricks to the nearest set brick, cap) as uint8."""
+    if not occ.any():
+        return np.full(occ.shape, cap, np.uint8)
+    d = ndimage.distance_transform_cdt(~occ, metric="chessboard")
+    return np.minimum(d, cap).astype(np.uint8)
 
 
 def _tsdf(kf, vol) -> np.ndarray:
@@ -41,13 +41,6 @@ def _tsdf(kf, vol) -> np.ndarray:
     if vol.desc.format == kf.KF_VOXEL_F16:
         return (raw.view(np.uint32) & 0xFFFF).astype(np.uint16).view(np.float16).astype(np.float32).reshape(n, n, n)
     return raw.view(np.float32)[0::2].reshape(n, n, n)
-
-
-def test_coarse_reference():
-    occ = np.zeros((8, 8, 8), bool)
-    occ[5, 0, 3] = True
-    c = coarse_of(occ)
-    assert c.shape == (2, 2, 2) and sorted(map(tuple, np.argwhere(c))) == [(1, 0, 0)]
 
 
 def test_exact_occupancy_reference():
@@ -59,6 +52,44 @@ def test_exact_occupancy_reference():
     f[:] = 1
     f[8, 8, 8] = -1
     assert len(np.argwhere(exact_occupancy(f))) == 8
+
+
+def test_brick_distance_reference_matches_brute_force():
+    rng = np.random.default_rng(3)
+    for nb, k in ((9, 4), (20, 2), (7, 0)):
+        occ = np.zeros((nb,) * 3, bool)
+        occ[tuple(rng.integers(0, nb, size=(3, k)))] = True
+        g = np.stack(np.meshgrid(*(np.arange(nb),) * 3, indexing="ij"), -1).reshape(-1, 1, 3)
+        pts = np.argwhere(occ).reshape(1, -1, 3)
+        if k:
+            brute = np.minimum(np.abs(g - pts).max(-1).min(-1), 16).reshape(occ.shape)
+        else:
+            brute = np.full(occ.shape, 16)
+        assert np.array_equal(brick_distance_of(occ), brute)
+    assert brick_distance_of(occ).max() == 16
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fmt", [0, 1], ids=["f32", "f16"])
+def test_brick_distance_matches_chessboard_transform(gpu, kf, fmt):
+    """nb = 25: two 16-column tiles per axis, the second partial; scattered
+    non-empty voxels so that distances reach the cap of 16."""
+    n = 200
+    d = vol_desc(n, 2.0, 0.05, fmt=fmt)
+    f = np.ones((n, n, n), np.float32)
+    for p in ((3, 5, 190), (130, 7, 9), (199, 199, 199), (64, 127, 100), (8, 8, 8)):
+        f[p] = 0.25
+    vol = kf.Volume(n, 2.0, 0.05, 128.0, fmt)
+    vol.upload_raw(fill_volume(vol.desc, f, 1.0))
+    occ = vol.occupancy()
+    assert np.array_equal(occ, exact_occupancy(_tsdf(kf, vol)))
+    dist = vol.brick_distance()
+    ref = brick_distance_of(occ)
+    assert dist.shape == (25, 25, 25) and ref.max() == 16 and ref.min() == 0
+    assert np.array_equal(dist, ref)
+    kf.General.Instance().InitVoxelArray(vol)
+    kf.synchronize()
+    assert np.all(vol.brick_distance() == 16)
 
 
 @pytest.mark.gpu
@@ -77,7 +108,6 @@ def test_rebuild_is_exact(gpu, kf, fmt):
     assert occ.shape == (8, 8, 8)
     assert 0 < ref.sum() < ref.size
     assert np.array_equal(occ, ref)
-    assert np.array_equal(vol.occupancy_coarse(), coarse_of(ref))
 
 
 @pytest.mark.gpu
@@ -86,7 +116,7 @@ def test_sweep_keeps_a_superset_and_reset_clears(gpu, kf, synth, small, fmt):
     k = synth.intrinsics()
     pr = synth.PRESETS["small"]
     vol = kf.Volume(pr.n, pr.extent, pr.trunc, pr.max_weight, fmt)
-    assert not vol.occupancy().any() and not vol.occupancy_coarse().any()
+    assert not vol.occupancy().any() and np.all(vol.brick_distance() == 16)
     integ = kf.VolumeIntegrator(k, integ_params())
     for i in range(5):
         live = kf.DeviceMaps(k, 1, True)
@@ -98,9 +128,9 @@ def test_sweep_keeps_a_superset_and_reset_clears(gpu, kf, synth, small, fmt):
     assert ref.sum() > 100
     assert not (ref & ~occ).any()            # never misses a brick
     assert (occ & ~ref).sum() <= 0.01 * ref.sum()  # and rarely keeps a stale one
-    assert np.array_equal(vol.occupancy_coarse(), coarse_of(occ))
+    assert np.array_equal(vol.brick_distance(), brick_distance_of(occ))
     vol.mark_modified()
     assert np.array_equal(vol.occupancy(), ref)
     kf.General.Instance().InitVoxelArray(vol)
     kf.synchronize()
-    assert not vol.occupancy().any() and not vol.occupancy_coarse().any()
+    assert not vol.occupancy().any() and np.all(vol.brick_distance() == 16)
