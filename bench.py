@@ -181,7 +181,7 @@ def kernel_rooflines(kf, d, sr: dict, prior: bool) -> dict:
                         "upper_bound_bytes": float(n ** 3 * 2 * Bv + P * 4)}
     S_bar = float(sr["steps"].mean() / P)
     ray_bytes = P * (S_bar * (Bv // 2) + 24)
-    t = float(ms[:, 4].mean())
+    t = float(ms[:, 5].mean())
     c = sr["counters"].mean(0)
     out["raycast"] = {"kernel": f"raycast<{'FP16' if Bv == 4 else 'FP32'}>", "algorithmic_bytes": ray_bytes,
                       "avg_launch_ms": t, "mean_steps_per_ray": S_bar,
@@ -341,7 +341,8 @@ def main() -> int:
         "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in res.items()},
         "observed_voxels_after_stage_run": sr["observed_voxels"],
-        "stages_ms": dict(zip(["preprocess", "guess+icp", "brick_classes", "sweep", "raycast", "model_pyramid"],
+        "stages_ms": dict(zip(["preprocess", "guess+icp", "brick_classes", "sweep", "brick_distances", "raycast",
+                               "pose_log"],
                               [round(float(x), 4) for x in stage])),
     }
     if "plain" in res:

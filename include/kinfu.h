@@ -418,10 +418,11 @@ kf_status kf_pipeline_get_stream(kf_pipeline* p, kf_stream* s);
 /* Stage-timing mode (eager launches, hipEvents on the pipeline stream, voxel and
  * march-step counters; frames are processed in order without deferral or overlap).
  * Times (ms) of the last processed frame:
- * [preprocess, guess+icp, raylen tiles + brick classes, sweep kernel, raycast kernel,
- *  model pyramid + pose log]; the sweep and raycast entries are the kernels' own
- * start-to-end times, the others the time between events around the stage. */
-#define KF_STAGES 6
+ * [preprocess, guess+icp, raylen tiles + brick classes, sweep kernel, brick
+ *  distances, raycast kernel (with the model pyramid), pose log]; the sweep and
+ * raycast entries are the kernels' own start-to-end times, the others the time
+ * between events around the stage. */
+#define KF_STAGES 7
 kf_status kf_pipeline_set_stage_timing(kf_pipeline* p, int32_t on);
 /* Stage-timing counters of the last frame: [voxels updated by the sweep, raycast
  * march steps, raycast samples that loaded their eight corners, raycast sample

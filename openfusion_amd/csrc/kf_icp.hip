@@ -580,8 +580,43 @@ __global__ __launch_bounds__(kStepThreads) void icp_step(IcpArgs a, StepArgs S) 
 // End of tracking (one block): solves the last step's slots, writes the output
 // pose and the statistics (iterations = updates applied), copies them to the
 // caller's statistics.  `steps` = 0: no step ran; the guess is the result.
+// Acceptance test of the finished call (IcpGate).  The window's median: lane i
+// ranks value i (ties broken by index) and the lanes of the middle rank(s) publish
+// their values, one pass instead of a serial sort.  Every thread of the block calls it.
+__device__ void gate_decide(const IcpGate& g, const kf_icp_stats* sp, const kf_pose* pose) {
+  __shared__ double v[kGateMaxWindow], mid[2];
+  __shared__ int s_fail;
+  const int i = threadIdx.x;
+  const int n = min(*g.count, kGateMaxWindow);
+  if (i < n) v[i] = g.window[i];
+  __syncthreads();
+  if (i < n) {
+    const double x = v[i];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) rank += (v[j] < x || (v[j] == x && j < i)) ? 1 : 0;
+    if (rank == n / 2) mid[1] = x;
+    if ((n & 1) == 0 && rank == n / 2 - 1) mid[0] = x;
+  }
+  __syncthreads();
+  if (i == 0) {
+    bool fail = sp->status != 0 || sp->iterations == 0 || !(sp->residual_rms == sp->residual_rms);
+    if (!fail && sp->candidates > 0 && (double)sp->inliers < (double)g.min_frac * (double)sp->candidates)
+      fail = true;
+    if (!fail && n >= g.window_min) {
+      const double med = (n & 1) ? mid[1] : 0.5 * (mid[0] + mid[1]);
+      if (sp->residual_rms > (double)g.ratio * med) fail = true;
+    }
+    *g.used = 1;
+    *g.fell_back = fail ? 1 : 0;
+    g.fallback->skip = fail ? 0 : 1;
+    s_fail = fail ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_fail) copy_pose(pose, g.accept_out);
+}
+
 __global__ __launch_bounds__(kSolveThreads) void icp_finish(IcpArgs a, StepArgs S, int steps, kf_pose* out,
-                                                            kf_icp_stats* user) {
+                                                            kf_icp_stats* user, IcpGate gate, int gated) {
   __shared__ double sum[kTerms + 2], Tin[16], Tnew[16];
   __shared__ kf_posef s_wc, s_cw;
   const int tid = threadIdx.x;
@@ -624,6 +659,10 @@ __global__ __launch_bounds__(kSolveThreads) void icp_finish(IcpArgs a, StepArgs 
       c.skip = 0;
       *user = c;
     }
+  }
+  if (gated) {  // uniform
+    __syncthreads();
+    gate_decide(gate, S.st, out);
   }
 }
 
@@ -855,13 +894,14 @@ kf_status check_maps(const kf_tracker* t, const kf_frame_maps& live, const kf_fr
 
 kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_frame_maps& model,
                            const kf_pose* d_guess, kf_pose* d_out, kf_icp_stats* d_stats,
-                           hipStream_t s, bool single_launch) {
+                           hipStream_t s, bool single_launch, const IcpGate* gate) {
   KF_CHECK_ARG(d_guess && d_out, "guess/out pose is NULL");
   const kf_icp_params& p = t->params;
   KF_TRY(check_maps(t, live, model, p.levels));
   kf_icp_stats* st = t->d_stats_tmp;
   float* slots[2] = {t->d_partials, t->d_partials + (size_t)t->max_blocks * kSlot};
   if (single_launch || p.single_launch) {
+    KF_CHECK_ARG(gate == nullptr, "an acceptance test needs the multi-launch call");
     ChainArgs C = {};
     C.levels = p.levels;
     int grid = 1;
@@ -925,7 +965,9 @@ kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_fr
   S.slots_in = slots[(step + 1) & 1];
   S.nb_in = prev_blocks;
   S.level_in = prev_level;
-  hipLaunchKernelGGL(icp_finish, dim3(1), dim3(kSolveThreads), 0, s, a, S, step, d_out, d_stats);
+  const IcpGate g = gate ? *gate : IcpGate{};
+  hipLaunchKernelGGL(icp_finish, dim3(1), dim3(kSolveThreads), 0, s, a, S, step, d_out, d_stats, g,
+                     gate ? 1 : 0);
   return check_launch("icp_finish");
 }
 

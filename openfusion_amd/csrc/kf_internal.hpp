@@ -115,14 +115,32 @@ constexpr size_t kRayTiles8 = (size_t)(kMaxImage / 8) * (kMaxImage / 8);
 constexpr size_t kRayTiles = kRayTiles8 + (size_t)(kMaxImage / 32) * (kMaxImage / 32);
 kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_raycast_params& p,
                          const kf_pose* d_pose, kf_frame_maps& model, unsigned long long* d_steps,
-                         hipStream_t s, hipEvent_t after_raycast = nullptr,
+                         hipStream_t s, hipEvent_t after_distances = nullptr,
                          unsigned long long* d_diag = nullptr, hipEvent_t kernel_start = nullptr,
                          hipEvent_t kernel_stop = nullptr);
+// Acceptance test of a warm-started tracking call, evaluated at the end of that
+// call: it fails when the call reports an error, ran no iteration or has a NaN
+// residual, when fewer than min_frac of the candidates are inliers, or (once
+// window_min residuals are known) when its residual exceeds ratio x the median of
+// the window.  Passed: *accept_out <- the call's pose and fallback->skip = 1;
+// failed: fallback->skip = 0.  *used <- 1, *fell_back <- failed.
+constexpr int kGateMaxWindow = 64;
+struct IcpGate {
+  const double* window;    // recent accepted residuals
+  const int32_t* count;    // how many of them are valid (<= kGateMaxWindow)
+  kf_icp_stats* fallback;  // skip flag of the fallback call
+  kf_pose* accept_out;
+  int32_t* used;
+  int32_t* fell_back;
+  float ratio, min_frac;
+  int window_min;
+};
 // single_launch: the whole call as one launch (also when the tracker's parameters
 // do not ask for it); used where most calls are skipped by the statistics' flag.
+// gate (multi-launch calls only): evaluated by the call's last kernel.
 kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_frame_maps& model,
                            const kf_pose* d_guess, kf_pose* d_out, kf_icp_stats* d_stats,
-                           hipStream_t s, bool single_launch = false);
+                           hipStream_t s, bool single_launch = false, const IcpGate* gate = nullptr);
 kf_status launch_icp_reduce(kf_tracker* t, int level, const kf_frame_maps& live,
                             const kf_frame_maps& model, const kf_pose* d_est, double* d_sums,
                             uint8_t* d_mask, hipStream_t s);

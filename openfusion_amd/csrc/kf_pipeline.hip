@@ -28,6 +28,7 @@ enum PoseSlot { kCur = 0, kPrev1 = 1, kPrev2 = 2, kGuess = 3, kGuessPrior = 4, k
 constexpr int kCounters = 6;
 constexpr int kWindow = 31;      // accepted residuals kept for the running median
 constexpr int kWindowMin = 5;    // residual test starts once this many are known
+static_assert(kWindow <= kf::kGateMaxWindow, "residual window larger than the acceptance test handles");
 
 // Device-side state of the learned prior in the pipeline.
 struct PriorState {
@@ -79,19 +80,6 @@ struct kf_pipeline {
 
 namespace kf {
 namespace {
-
-// End of frame: log the frame's pose and shift the history (k-2 <- k-1 <- k).
-__global__ void pose_shift(kf_pose* poses, double* traj, int* count) {
-  if (threadIdx.x < 16) {
-    const int c = *count;
-    if (c < KF_TRAJ_CAPACITY) traj[(size_t)c * 16 + threadIdx.x] = poses[kCur].T_wc[threadIdx.x];
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  *count = *count + 1;
-  poses[kPrev2] = poses[kPrev1];
-  poses[kPrev1] = poses[kCur];
-}
 
 // slot 0 <- slot 1 (previous frame's 160x120 depth), slot 1 <- current level-2 depth.
 __global__ void prior_stage_inputs(const float* l2, float* frames, int n) {
@@ -161,61 +149,40 @@ __global__ void prior_guess(PriorState* st, kf_pose* poses) {
   poses[kGuessPrior] = np;
 }
 
-// Accept the warm-started ICP or arm the plain ICP (its skip flag).
-__global__ void prior_decide(PriorState* st, const kf_icp_stats* sp, kf_icp_stats* plain, kf_pose* poses,
-                             float ratio, float min_frac) {
-  // Running median of the window: lane i ranks value i (ties broken by index), the
-  // lanes of the middle rank(s) publish their values — one pass instead of a serial
-  // sort.  Launched with 64 threads; kWindow <= 64.
-  __shared__ double v[kWindow], mid[2];
-  const int i = threadIdx.x;
-  const int n = st->win_count;
-  if (i < n) v[i] = st->window[i];
-  __syncthreads();
-  if (i < n) {
-    const double x = v[i];
-    int rank = 0;
-    for (int j = 0; j < n; ++j) rank += (v[j] < x || (v[j] == x && j < i)) ? 1 : 0;
-    if (rank == n / 2) mid[1] = x;
-    if ((n & 1) == 0 && rank == n / 2 - 1) mid[0] = x;
+// Frame end.  With a prior: record the accepted residual and the network's next
+// motion input (scaled twist of T_{k-1}^-1 T_k).  Then log the frame's pose and
+// shift the history (k-2 <- k-1 <- k).
+__global__ void frame_end(PriorState* st, const kf_icp_stats* sp, const kf_icp_stats* plain, kf_pose* poses,
+                          double* traj, int* count) {
+  if (st && threadIdx.x == 0) {
+    const double res = st->fell_back ? plain->residual_rms : sp->residual_rms;
+    if (res == res && res > 0.0) {
+      st->window[st->win_head] = res;
+      st->win_head = (st->win_head + 1) % kWindow;
+      if (st->win_count < kWindow) st->win_count += 1;
+    }
+    st->fallbacks += st->fell_back;
+    st->frames += 1;
+    double Ti[16], D[16];
+    rigid_inverse(poses[kPrev1].T_wc, Ti);
+    mat4_mul(Ti, poses[kCur].T_wc, D);
+    const double R[9] = {D[0], D[1], D[2], D[4], D[5], D[6], D[8], D[9], D[10]};
+    double w[3];
+    R_to_rotvec(R, w);
+    for (int i = 0; i < 3; ++i) {
+      st->prev_twist[i] = (float)(w[i] / KF_PRIOR_ROT_SCALE);
+      st->prev_twist[3 + i] = (float)(D[i * 4 + 3] / KF_PRIOR_TRANS_SCALE);
+    }
+  }
+  if (threadIdx.x < 16) {
+    const int c = *count;
+    if (c < KF_TRAJ_CAPACITY) traj[(size_t)c * 16 + threadIdx.x] = poses[kCur].T_wc[threadIdx.x];
   }
   __syncthreads();
-  if (i != 0) return;
-  bool fail = sp->status != 0 || sp->iterations == 0 || !(sp->residual_rms == sp->residual_rms);
-  if (!fail && sp->candidates > 0 && (double)sp->inliers < (double)min_frac * (double)sp->candidates) fail = true;
-  if (!fail && n >= kWindowMin) {
-    const double med = (n & 1) ? mid[1] : 0.5 * (mid[0] + mid[1]);
-    if (sp->residual_rms > (double)ratio * med) fail = true;
-  }
-  st->used_prior = 1;
-  st->fell_back = fail ? 1 : 0;
-  if (!fail) poses[kCur] = poses[kPriorOut];
-  plain->skip = fail ? 0 : 1;
-}
-
-// Frame end with a prior: record the accepted residual and the network's next
-// motion input (scaled twist of T_{k-1}^-1 T_k).
-__global__ void prior_post(PriorState* st, const kf_icp_stats* sp, const kf_icp_stats* plain,
-                           const kf_pose* poses) {
   if (threadIdx.x != 0) return;
-  const double res = st->fell_back ? plain->residual_rms : sp->residual_rms;
-  if (res == res && res > 0.0) {
-    st->window[st->win_head] = res;
-    st->win_head = (st->win_head + 1) % kWindow;
-    if (st->win_count < kWindow) st->win_count += 1;
-  }
-  st->fallbacks += st->fell_back;
-  st->frames += 1;
-  double Ti[16], D[16];
-  rigid_inverse(poses[kPrev1].T_wc, Ti);
-  mat4_mul(Ti, poses[kCur].T_wc, D);
-  const double R[9] = {D[0], D[1], D[2], D[4], D[5], D[6], D[8], D[9], D[10]};
-  double w[3];
-  R_to_rotvec(R, w);
-  for (int i = 0; i < 3; ++i) {
-    st->prev_twist[i] = (float)(w[i] / KF_PRIOR_ROT_SCALE);
-    st->prev_twist[3 + i] = (float)(D[i * 4 + 3] / KF_PRIOR_TRANS_SCALE);
-  }
+  *count = *count + 1;
+  poses[kPrev2] = poses[kPrev1];
+  poses[kPrev1] = poses[kCur];
 }
 
 // Live maps of frame f (the two sets alternate).
@@ -249,31 +216,41 @@ kf_status record_track(kf_pipeline* p, kf_frame_maps& live, hipStream_t s) {
   if (p->prior) {
     hipLaunchKernelGGL(prior_guess, dim3(1), dim3(64), 0, s, p->d_prior, p->d_poses);
     KF_TRY(check_launch("prior_guess"));
+    // the warm-started call ends with the acceptance test: accepted, its pose is the
+    // frame's and the fallback's skip flag is set
+    char* st = reinterpret_cast<char*>(p->d_prior);  // device addresses of PriorState members
+    IcpGate g;
+    g.window = reinterpret_cast<const double*>(st + offsetof(PriorState, window));
+    g.count = reinterpret_cast<const int32_t*>(st + offsetof(PriorState, win_count));
+    g.fallback = &p->d_stats[0];
+    g.accept_out = &p->d_poses[kCur];
+    g.used = reinterpret_cast<int32_t*>(st + offsetof(PriorState, used_prior));
+    g.fell_back = reinterpret_cast<int32_t*>(st + offsetof(PriorState, fell_back));
+    g.ratio = d.fallback_residual_ratio;
+    g.min_frac = d.fallback_min_inlier_frac;
+    g.window_min = kWindowMin;
     KF_TRY(launch_icp_track(p->tracker_prior, live, p->model, &p->d_poses[kGuessPrior], &p->d_poses[kPriorOut],
-                            &p->d_stats[1], s));
-    hipLaunchKernelGGL(prior_decide, dim3(1), dim3(64), 0, s, p->d_prior, (const kf_icp_stats*)&p->d_stats[1],
-                       &p->d_stats[0], p->d_poses, d.fallback_residual_ratio, d.fallback_min_inlier_frac);
-    KF_TRY(check_launch("prior_decide"));
-    // the fallback: plain ICP from the constant-velocity guess, skipped (by the flag
-    // prior_decide sets) in most frames, hence one launch for the whole call
+                            &p->d_stats[1], s, false, &g));
+    // the fallback: plain ICP from the constant-velocity guess, skipped in most
+    // frames, hence one launch for the whole call
     KF_TRY(launch_icp_track(p->tracker, live, p->model, &p->d_poses[kGuess], &p->d_poses[kCur], &p->d_stats[0], s,
                             true));
-    hipLaunchKernelGGL(prior_post, dim3(1), dim3(64), 0, s, p->d_prior, (const kf_icp_stats*)&p->d_stats[1],
-                       (const kf_icp_stats*)&p->d_stats[0], (const kf_pose*)p->d_poses);
-    KF_TRY(check_launch("prior_post"));
   } else {
     KF_TRY(launch_icp_track(p->tracker, live, p->model, &p->d_poses[kGuess], &p->d_poses[kCur], &p->d_stats[0], s));
   }
   return KF_OK;
 }
 
-kf_status record_pose_shift(kf_pipeline* p, hipStream_t s) {
-  hipLaunchKernelGGL(pose_shift, dim3(1), dim3(64), 0, s, p->d_poses, p->d_traj, p->d_traj_count);
-  return check_launch("pose_shift");
+// End of a frame; tracked: the frame was tracked (not the first one).
+kf_status record_frame_end(kf_pipeline* p, hipStream_t s, bool tracked) {
+  hipLaunchKernelGGL(frame_end, dim3(1), dim3(64), 0, s, p->prior && tracked ? p->d_prior : nullptr,
+                     (const kf_icp_stats*)&p->d_stats[1], (const kf_icp_stats*)&p->d_stats[0], p->d_poses,
+                     p->d_traj, p->d_traj_count);
+  return check_launch("frame_end");
 }
 
 // Integration and raycast of the previous tracked frame (its pose is kPrev1 once
-// pose_shift has run).
+// frame_end has run).
 kf_status record_finish(kf_pipeline* p, const kf_frame_maps& live, hipStream_t s) {
   const kf_pipeline_desc& d = p->d;
   KF_TRY(launch_integrate(p->vol, d.intr, d.integ, live.raylen, &p->d_poses[kPrev1], nullptr, nullptr, s));
@@ -291,7 +268,7 @@ kf_status record_call(kf_pipeline* p, int k, bool finish) {
     if (finish) KF_TRY(record_finish(p, live_of(p, k - 1), s));
     KF_TRY(record_front(p, live_of(p, k), s));
     KF_TRY(record_track(p, live_of(p, k), s));
-    return record_pose_shift(p, s);
+    return record_frame_end(p, s, true);
   }
   KF_HIP(hipEventRecord(p->fork_ev, s));
   KF_HIP(hipStreamWaitEvent(p->side, p->fork_ev, 0));
@@ -300,7 +277,7 @@ kf_status record_call(kf_pipeline* p, int k, bool finish) {
   KF_HIP(hipEventRecord(p->join_ev, p->side));
   KF_HIP(hipStreamWaitEvent(s, p->join_ev, 0));
   KF_TRY(record_track(p, live_of(p, k), s));
-  return record_pose_shift(p, s);
+  return record_frame_end(p, s, true);
 }
 
 // One frame in stage-timing mode (eager, one stream, nothing deferred): events
@@ -329,8 +306,9 @@ kf_status record_timed_frame(kf_pipeline* p, int k) {
   KF_HIP(hipEventRecord(p->ev[4], s));
   KF_TRY(launch_raycast(p->vol, d.intr, d.ray, &p->d_poses[kCur], p->model, &p->d_counters[1], s, p->ev[5],
                         &p->d_counters[2], p->kev[2], p->kev[3]));
-  KF_TRY(record_pose_shift(p, s));
   KF_HIP(hipEventRecord(p->ev[6], s));
+  KF_TRY(record_frame_end(p, s, true));
+  KF_HIP(hipEventRecord(p->ev[7], s));
   return KF_OK;
 }
 
@@ -514,7 +492,7 @@ extern "C" kf_status kf_pipeline_process(kf_pipeline* p, const uint16_t* d_depth
     kf_frame_maps& live = live_of(p, 0);
     KF_TRY(launch_preprocess(p->d.intr, p->d.pre, p->d_depth, live, s));
     if (p->prior) KF_TRY(stage_prior_inputs(p, live, s));
-    KF_TRY(record_pose_shift(p, s));
+    KF_TRY(record_frame_end(p, s, false));
     p->pending = true;
   } else if (p->stage_timing) {
     KF_TRY(flush(p));
@@ -523,7 +501,7 @@ extern "C" kf_status kf_pipeline_process(kf_pipeline* p, const uint16_t* d_depth
     for (int i = 0; i < KF_STAGES; ++i) KF_HIP(hipEventElapsedTime(&p->stage_ms[i], p->ev[i], p->ev[i + 1]));
     // the sweep and raycast stages are their kernels' own start-to-end times
     KF_HIP(hipEventElapsedTime(&p->stage_ms[3], p->kev[0], p->kev[1]));
-    KF_HIP(hipEventElapsedTime(&p->stage_ms[4], p->kev[2], p->kev[3]));
+    KF_HIP(hipEventElapsedTime(&p->stage_ms[5], p->kev[2], p->kev[3]));
   } else if (!p->d.use_graph || !p->pending) {
     KF_TRY(record_call(p, k, p->pending));
     p->pending = true;

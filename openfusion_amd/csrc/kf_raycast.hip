@@ -28,6 +28,10 @@ struct RayArgs {
   const void* vox;
   const kf_pose* pose;
   kf_level_maps out;
+  // model pyramid: level l >= 1 takes every 2^l-th sample of level 0, written by the
+  // lane of that level-0 pixel
+  kf_level_maps sub[KF_MAX_LEVELS];  // [1], [2] used
+  int levels;
   unsigned long long* steps;
   unsigned long long* diag;  // optional [samples that loaded corners, batches, skipped steps, 64 x wave max batches]
   const unsigned long long* occ;  // brick occupancy map or NULL
@@ -375,6 +379,23 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
     a.out.nx[idx] = nrm.x;
     a.out.ny[idx] = nrm.y;
     a.out.nz[idx] = nrm.z;
+#pragma unroll
+    for (int l = 1; l < KF_MAX_LEVELS; ++l) {
+      const int m = (1 << l) - 1;
+      if (l < a.levels && !(x & m) && !(y & m)) {
+        const kf_level_maps& o = a.sub[l];
+        const int xl = x >> l, yl = y >> l;
+        if (xl < o.width && yl < o.height) {
+          const int j = yl * o.width + xl;
+          o.vx[j] = vert.x;
+          o.vy[j] = vert.y;
+          o.vz[j] = vert.z;
+          o.nx[j] = nrm.x;
+          o.ny[j] = nrm.y;
+          o.nz[j] = nrm.z;
+        }
+      }
+    }
   }
   if (a.diag) {  // diagnostics: one device atomic per block and counter
     __shared__ int dred[4][4];
@@ -406,38 +427,11 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
   }
 }
 
-// Model pyramid: level l (l >= 1) takes every 2^l-th sample of level 0, all six planes.
-struct SubArgs {
-  kf_level_maps l0;
-  kf_level_maps lv[KF_MAX_LEVELS];  // [1], [2] used
-  int levels;
-  int begin2;  // first thread index of level 2
-};
-
-__global__ __launch_bounds__(256) void subsample_maps(SubArgs s) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int l = 1;
-  if (s.levels > 2 && i >= s.begin2) {
-    l = 2;
-    i -= s.begin2;
-  }
-  const kf_level_maps& o = s.lv[l];
-  if (i >= o.width * o.height) return;
-  const int y = i / o.width, x = i - y * o.width;
-  const int j = (y << l) * s.l0.width + (x << l);
-  o.vx[i] = s.l0.vx[j];
-  o.vy[i] = s.l0.vy[j];
-  o.vz[i] = s.l0.vz[j];
-  o.nx[i] = s.l0.nx[j];
-  o.ny[i] = s.l0.ny[j];
-  o.nz[i] = s.l0.nz[j];
-}
-
 }  // namespace
 
 kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_raycast_params& p,
                          const kf_pose* d_pose, kf_frame_maps& model, unsigned long long* d_steps,
-                         hipStream_t s, hipEvent_t after_raycast, unsigned long long* d_diag,
+                         hipStream_t s, hipEvent_t after_distances, unsigned long long* d_diag,
                          hipEvent_t kernel_start, hipEvent_t kernel_stop) {
   KF_CHECK_ARG(d_pose != nullptr, "pose is NULL");
   KF_CHECK_ARG(model.levels >= 1 && model.levels <= KF_MAX_LEVELS, "model levels must be 1..3");
@@ -453,10 +447,13 @@ kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_r
   a.vox = v->d_voxels;
   a.pose = d_pose;
   a.out = model.level[0];
+  a.levels = model.levels;
+  for (int l = 0; l < KF_MAX_LEVELS; ++l) a.sub[l] = l < model.levels ? model.level[l] : kf_level_maps{};
   a.steps = d_steps;
   a.diag = d_diag;
   KF_TRY(ensure_occupancy(v, s));
   KF_TRY(launch_brick_distance(v, s));
+  if (after_distances) KF_HIP(hipEventRecord(after_distances, s));
   a.occ = v->d_occ;
   a.bdist = v->d_bdist;
   a.occ_nb = v->occ_nb;
@@ -489,19 +486,7 @@ kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_r
     hipExtLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, kernel_start, kernel_stop, 0, a);
   else
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, a);
-  KF_TRY(check_launch("raycast"));
-  if (after_raycast) KF_HIP(hipEventRecord(after_raycast, s));
-  if (model.levels > 1) {
-    SubArgs sa;
-    sa.l0 = model.level[0];
-    sa.levels = model.levels;
-    for (int l = 1; l < model.levels; ++l) sa.lv[l] = model.level[l];
-    sa.begin2 = model.level[1].width * model.level[1].height;
-    const int total = sa.begin2 + (model.levels > 2 ? model.level[2].width * model.level[2].height : 0);
-    hipLaunchKernelGGL(subsample_maps, dim3(div_up(total, 256)), dim3(256), 0, s, sa);
-    KF_TRY(check_launch("subsample_maps"));
-  }
-  return KF_OK;
+  return check_launch("raycast");
 }
 
 }  // namespace kf

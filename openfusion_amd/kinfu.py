@@ -104,7 +104,7 @@ class FrameStats(C.Structure):
                 ("prior_frames", C.c_int32), ("prior_fallbacks", C.c_int32)]
 
 
-STAGES = 6  # preprocess, guess+icp, brick classes, sweep kernel, raycast kernel, model pyramid + pose log
+STAGES = 7  # preprocess, guess+icp, brick classes, sweep kernel, brick distances, raycast kernel, pose log
 
 
 # ---------------------------------------------------------------------------
