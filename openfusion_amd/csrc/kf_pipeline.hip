@@ -62,6 +62,7 @@ struct kf_pipeline {
   int frame = 0;
   hipGraph_t graph[2] = {};          // frame graphs for even / odd frames (live maps alternate)
   hipGraphExec_t exec[2] = {};
+  hipGraphNode_t copy_node[2] = {};  // each graph's depth copy (its source is set per launch)
   bool stage_timing = false;
   hipEvent_t ev[KF_STAGES + 1] = {};
   hipEvent_t kev[4] = {};  // own start / end of the sweep kernel, then of the raycast kernel
@@ -257,14 +258,36 @@ kf_status record_finish(kf_pipeline* p, const kf_frame_maps& live, hipStream_t s
   return launch_raycast(p->vol, d.intr, d.ray, &p->d_poses[kPrev1], p->model, nullptr, s);
 }
 
+// Copies the caller's depth frame into the staging buffer on stream st while it is
+// captured, and returns the copy's graph node (its source is replaced per launch).
+kf_status record_depth_copy(kf_pipeline* p, const uint16_t* src, hipStream_t st, hipGraphNode_t* node) {
+  const size_t bytes = (size_t)p->d.intr.width * p->d.intr.height * sizeof(uint16_t);
+  KF_HIP(hipMemcpyAsync(p->d_depth, src, bytes, hipMemcpyDeviceToDevice, st));
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  const hipGraphNode_t* deps = nullptr;
+  size_t nd = 0;
+  KF_HIP(hipStreamGetCaptureInfo_v2(st, &cs, nullptr, nullptr, &deps, &nd));
+  if (cs != hipStreamCaptureStatusActive || nd != 1 || !deps) {
+    set_error("depth copy: the capture did not record one node");
+    return KF_ERR_HIP;
+  }
+  *node = deps[0];
+  return KF_OK;
+}
+
 // The call for frame k (k >= 1): [previous frame's integrate + raycast on the
 // pipeline stream] in parallel with [frame k's front part on the side stream], then
 // tracking and the pose log on the pipeline stream.  Without a prior the front
 // part is the preprocessing alone, which contends with the sweep and raycast for
-// the same units (measured slower in parallel), so then everything runs in order.
-kf_status record_call(kf_pipeline* p, int k, bool finish) {
+// the same units, and a graph with a second stream costs more than it saves then
+// (measured), so everything runs in order on the pipeline stream.  src (graph
+// capture only): the depth copy is recorded at the start of the front part's
+// stream.
+kf_status record_call(kf_pipeline* p, int k, bool finish, const uint16_t* src = nullptr,
+                      hipGraphNode_t* copy_node = nullptr) {
   hipStream_t s = p->stream;
   if (!p->prior) {
+    if (src) KF_TRY(record_depth_copy(p, src, s, copy_node));
     if (finish) KF_TRY(record_finish(p, live_of(p, k - 1), s));
     KF_TRY(record_front(p, live_of(p, k), s));
     KF_TRY(record_track(p, live_of(p, k), s));
@@ -272,6 +295,7 @@ kf_status record_call(kf_pipeline* p, int k, bool finish) {
   }
   KF_HIP(hipEventRecord(p->fork_ev, s));
   KF_HIP(hipStreamWaitEvent(p->side, p->fork_ev, 0));
+  if (src) KF_TRY(record_depth_copy(p, src, p->side, copy_node));
   if (finish) KF_TRY(record_finish(p, live_of(p, k - 1), s));
   KF_TRY(record_front(p, live_of(p, k), p->side));
   KF_HIP(hipEventRecord(p->join_ev, p->side));
@@ -328,6 +352,7 @@ void destroy_graph(kf_pipeline* p) {
     if (p->graph[i]) (void)hipGraphDestroy(p->graph[i]);
     p->exec[i] = nullptr;
     p->graph[i] = nullptr;
+    p->copy_node[i] = nullptr;
   }
 }
 
@@ -484,7 +509,9 @@ extern "C" kf_status kf_pipeline_process(kf_pipeline* p, const uint16_t* d_depth
   hipStream_t s = p->stream;
   const int k = p->frame;
   const size_t bytes = (size_t)p->d.intr.width * p->d.intr.height * sizeof(uint16_t);
-  KF_HIP(hipMemcpyAsync(p->d_depth, d_depth, bytes, hipMemcpyDeviceToDevice, s));
+  // graph replay: the frame graph copies the depth itself
+  const bool replay = k > 0 && !p->stage_timing && p->d.use_graph && p->pending;
+  if (!replay) KF_HIP(hipMemcpyAsync(p->d_depth, d_depth, bytes, hipMemcpyDeviceToDevice, s));
   // Voxels written outside the library since the last frame: rebuild the brick map
   // here, eagerly, so that it is never recorded into the frame graph.
   KF_TRY(ensure_occupancy(p->vol, s));
@@ -509,7 +536,7 @@ extern "C" kf_status kf_pipeline_process(kf_pipeline* p, const uint16_t* d_depth
     const int g = k & 1;
     if (!p->exec[g]) {
       KF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      kf_status st = record_call(p, k, true);
+      kf_status st = record_call(p, k, true, d_depth, &p->copy_node[g]);
       hipGraph_t gr = nullptr;
       hipError_t e = hipStreamEndCapture(s, &gr);
       if (st != KF_OK) {
@@ -524,6 +551,8 @@ extern "C" kf_status kf_pipeline_process(kf_pipeline* p, const uint16_t* d_depth
         return hip_fail(e, "hipGraphInstantiate");
       }
     }
+    KF_HIP(hipGraphExecMemcpyNodeSetParams1D(p->exec[g], p->copy_node[g], p->d_depth, d_depth, bytes,
+                                             hipMemcpyDeviceToDevice));
     KF_HIP(hipGraphLaunch(p->exec[g], s));
     p->pending = true;
   }
