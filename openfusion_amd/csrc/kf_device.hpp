@@ -135,6 +135,20 @@ __device__ __forceinline__ unsigned long long occ_zbits(int z) {
 // wave, uniform control flow).  Up to four brick columns receive the bits (the
 // column's own and, on a brick boundary, the one below in x and/or y); a word is
 // only written when bits are missing.
+// x returned through an empty asm statement: the compiler cannot see its range, so
+// it neither widens the arithmetic that uses it to 64 bits nor fuses a 24-bit
+// multiply and add into a 64-bit multiply-add.
+__device__ __forceinline__ unsigned opaque_u32(unsigned x) {
+  asm("" : "+v"(x));
+  return x;
+}
+
+// Element at byte offset off of a plane: a uniform base address plus one 32-bit
+// offset register, shared by every plane read at the same index.
+__device__ __forceinline__ float plane_at(const float* p, unsigned off) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + off);
+}
+
 // Brick occupancy map: one 64-bit word per brick column (bx, by), bit bz.
 struct OccMap {
   unsigned long long* words;  // nb * nb, bit bz

@@ -78,8 +78,11 @@ struct PxLive {
   f3 dc, nc;
 };
 
+// (byte offsets of a level's pixels fit in 32 bits: sides are at most kMaxImage)
 __device__ __forceinline__ PxModel load_model(const IcpArgs& a, int i) {
-  return PxModel{f3{a.mvx[i], a.mvy[i], a.mvz[i]}, f3{a.mnx[i], a.mny[i], a.mnz[i]}, a.lvz[i]};
+  const unsigned o = opaque_u32((unsigned)i * 4u);
+  return PxModel{f3{plane_at(a.mvx, o), plane_at(a.mvy, o), plane_at(a.mvz, o)},
+                 f3{plane_at(a.mnx, o), plane_at(a.mny, o), plane_at(a.mnz, o)}, plane_at(a.lvz, o)};
 }
 
 // Live pixel the model point projects to, or -1 (0 = not a candidate, 1 = rejected
@@ -99,8 +102,9 @@ __device__ __forceinline__ int project_model(const IcpArgs& a, const kf_posef& c
 }
 
 __device__ __forceinline__ PxLive load_live(const IcpArgs& a, int j) {
-  const int jj = j < 0 ? 0 : j;  // always in bounds; the result is unused when j < 0
-  return PxLive{f3{a.lvx[jj], a.lvy[jj], a.lvz[jj]}, f3{a.lnx[jj], a.lny[jj], a.lnz[jj]}};
+  const unsigned o = opaque_u32((unsigned)(j < 0 ? 0 : j) * 4u);  // in bounds; unused when j < 0
+  return PxLive{f3{plane_at(a.lvx, o), plane_at(a.lvy, o), plane_at(a.lvz, o)},
+                f3{plane_at(a.lnx, o), plane_at(a.lny, o), plane_at(a.lnz, o)}};
 }
 
 __device__ __forceinline__ int finish_row(const IcpArgs& a, const kf_posef& wc, const PxModel& q, const PxLive& l,
@@ -179,27 +183,63 @@ __device__ __forceinline__ void halve(float* s, float* c, bool upper) {
 }
 
 // Block partial: this block's share of the association (pixels bid*T + tid + k*nblk*T,
-// two per step with their loads in flight together), the 28 terms per thread,
-// the reduce-scatter wave reduction and the combine over the block's waves,
-// written as one slot: 28 (sum, error) pairs and the inlier / candidate counts.
-// The model samples of a thread's first two pixels in block_partial; they do not
+// K per round with their loads in flight together, consumed in index order), the
+// 28 terms per thread, the reduce-scatter wave reduction and the combine over the
+// block's waves, written as one slot: 28 (sum, error) pairs and the inlier /
+// candidate counts.  The sums do not depend on K.
+// The model samples of a thread's first K pixels in block_partial; they do not
 // depend on the pose, so a step can load them before it solves for the pose.
-struct PxPair {
-  PxModel q0, q1;
+template <int K>
+struct PxPre {
+  PxModel q[K];
 };
 
-template <int WAVES>
-__device__ __forceinline__ PxPair prefetch_models(const IcpArgs& a, int bid, int nblk) {
+template <int WAVES, int K>
+__device__ __forceinline__ PxPre<K> prefetch_models(const IcpArgs& a, int bid, int nblk) {
   constexpr int T = WAVES * kWave;
   const int P = a.W * a.H, stride = nblk * T;
-  const int i0 = bid * T + (int)threadIdx.x, i1 = i0 + stride;
+  const int i0 = bid * T + (int)threadIdx.x;
   const int j0 = i0 < P ? i0 : 0;  // in bounds; unused when the block has no pixel here
-  return PxPair{load_model(a, j0), load_model(a, i1 < P ? i1 : j0)};
+  PxPre<K> pre;
+#pragma unroll
+  for (int k = 0; k < K; ++k) pre.q[k] = load_model(a, i0 + k * stride < P ? i0 + k * stride : j0);
+  return pre;
 }
 
-template <bool COMP, int WAVES>
+// One round of a thread's association: pixels i0 + k * stride (k < K) with model
+// samples q, their live loads in flight together, the rows accumulated in order.
+template <bool COMP, int K>
+__device__ __forceinline__ void associate_round(const IcpArgs& a, const kf_posef& wc, const kf_posef& cw, int i0,
+                                                int stride, int P, const PxModel (&q)[K], float* s, float* c,
+                                                int& n_in, int& n_cand) {
+  int j[K], r[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    j[k] = project_model(a, cw, q[k], r[k]);
+    if (i0 + k * stride >= P) {  // no pixel: contributes nothing
+      j[k] = -1;
+      r[k] = 0;
+    }
+  }
+  PxLive l[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) l[k] = load_live(a, j[k]);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    float A[6], b;
+    r[k] = finish_row(a, wc, q[k], l[k], j[k], r[k], A, b);
+    if (a.mask && i0 + k * stride < P) a.mask[i0 + k * stride] = (uint8_t)(r[k] == 2);
+    n_cand += r[k] != 0 ? 1 : 0;
+    if (r[k] == 2) {
+      ++n_in;
+      accumulate<COMP>(s, c, A, b);
+    }
+  }
+}
+
+template <bool COMP, int WAVES, int K = 2, bool PRE = false>
 __device__ __forceinline__ void block_partial(const IcpArgs& a, const kf_posef& wc, const kf_posef& cw, int bid,
-                                              int nblk, float* slot, const PxPair* pre = nullptr) {
+                                              int nblk, float* slot, PxPre<K> pre = {}) {
   __shared__ float red_s[WAVES][kPad];
   __shared__ float red_c[WAVES][kPad];
   __shared__ int red_n[WAVES][2];
@@ -214,38 +254,16 @@ __device__ __forceinline__ void block_partial(const IcpArgs& a, const kf_posef& 
   int n_in = 0, n_cand = 0;
   const int P = a.W * a.H;
   const int stride = nblk * T;
-  bool first = pre != nullptr;
-  for (int i0 = bid * T + tid; i0 < P; i0 += 2 * stride) {
-    const int i1 = i0 + stride;
-    const bool has1 = i1 < P;
-    const PxModel q0 = first ? pre->q0 : load_model(a, i0);
-    const PxModel q1 = first ? pre->q1 : load_model(a, has1 ? i1 : i0);
-    first = false;
-    int r0, r1;
-    const int j0 = project_model(a, cw, q0, r0);
-    int j1 = project_model(a, cw, q1, r1);
-    if (!has1) {
-      j1 = -1;
-      r1 = 0;
-    }
-    const PxLive l0 = load_live(a, j0);
-    const PxLive l1 = load_live(a, j1);
-    float A0[6], b0, A1[6], b1;
-    r0 = finish_row(a, wc, q0, l0, j0, r0, A0, b0);
-    r1 = finish_row(a, wc, q1, l1, j1, r1, A1, b1);
-    if (a.mask) {
-      a.mask[i0] = (uint8_t)(r0 == 2);
-      if (has1) a.mask[i1] = (uint8_t)(r1 == 2);
-    }
-    n_cand += (r0 != 0 ? 1 : 0) + (r1 != 0 ? 1 : 0);
-    if (r0 == 2) {
-      ++n_in;
-      accumulate<COMP>(s, c, A0, b0);
-    }
-    if (r1 == 2) {
-      ++n_in;
-      accumulate<COMP>(s, c, A1, b1);
-    }
+  int i0 = bid * T + tid;
+  if (PRE && i0 < P) {  // first round from the prefetched samples
+    associate_round<COMP, K>(a, wc, cw, i0, stride, P, pre.q, s, c, n_in, n_cand);
+    i0 += K * stride;
+  }
+  for (; i0 < P; i0 += K * stride) {
+    PxModel q[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) q[k] = load_model(a, i0 + k * stride < P ? i0 + k * stride : i0);
+    associate_round<COMP, K>(a, wc, cw, i0, stride, P, q, s, c, n_in, n_cand);
   }
 
   // Wave reduction by recursive halving (reduce-scatter): five pairings, each
@@ -310,9 +328,28 @@ __device__ __forceinline__ void block_partial(const IcpArgs& a, const kf_posef& 
 // 2j, 2j + 1 as (hi, lo) pairs; j = 14: the counts) with kFinUnroll loads in
 // flight; the group totals are combined in a fixed order (lane shuffles within a
 // wave, then the waves in order).
+// Split in two so that a caller can issue the first round of slot loads before
+// other loads (loads complete in issue order) and consume them later.
 template <int THREADS>
-__device__ __forceinline__ void sum_slots(const float* slots, int nb, double* sum) {
-  constexpr int G = THREADS / 16, WV = THREADS / kWave;
+struct SlotSum {
+  static constexpr int G = THREADS / 16;
+  float4 v[kFinUnroll];
+  // The first round of loads (blocks g, g + G, ..., g + (kFinUnroll - 1) G; nb >= 1).
+  // Unconditional, from clamped in-bounds addresses, so that nothing waits for
+  // them here; finish() drops the out-of-range ones.
+  __device__ __forceinline__ void load(const float* slots, int nb) {
+    const int tid = threadIdx.x;
+    const int j = tid & 15, g = tid >> 4;
+    const float4* base = reinterpret_cast<const float4*>(slots) + (j < 15 ? j : 0);
+#pragma unroll
+    for (int u = 0; u < kFinUnroll; ++u) v[u] = base[(size_t)min(g + G * u, nb - 1) * (kSlot / 4)];
+  }
+  __device__ __forceinline__ void finish(const float* slots, int nb, double* sum);
+};
+
+template <int THREADS>
+__device__ __forceinline__ void SlotSum<THREADS>::finish(const float* slots, int nb, double* sum) {
+  constexpr int WV = THREADS / kWave;
   __shared__ double fin[WV][kTerms + 2];
   const int tid = threadIdx.x;
   const int j = tid & 15, g = tid >> 4;
@@ -320,14 +357,13 @@ __device__ __forceinline__ void sum_slots(const float* slots, int nb, double* su
   if (j < 15) {
     const float4* base = reinterpret_cast<const float4*>(slots) + j;
     for (int b0 = g; b0 < nb; b0 += G * kFinUnroll) {
-      float4 v[kFinUnroll];
+      if (b0 != g) {
 #pragma unroll
-      for (int u = 0; u < kFinUnroll; ++u) {
-        const int bb = b0 + G * u;
-        v[u] = bb < nb ? base[(size_t)bb * (kSlot / 4)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int u = 0; u < kFinUnroll; ++u) v[u] = base[(size_t)min(b0 + G * u, nb - 1) * (kSlot / 4)];
       }
 #pragma unroll
       for (int u = 0; u < kFinUnroll; ++u) {
+        if (b0 + G * u >= nb) v[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (j < 14) {
           a0 = (a0 + (double)v[u].x) + (double)v[u].y;
           a1 = (a1 + (double)v[u].z) + (double)v[u].w;
@@ -354,6 +390,13 @@ __device__ __forceinline__ void sum_slots(const float* slots, int nb, double* su
     sum[tid] = t;
   }
   __syncthreads();
+}
+
+template <int THREADS>
+__device__ __forceinline__ void sum_slots(const float* slots, int nb, double* sum) {
+  SlotSum<THREADS> L;
+  L.load(slots, nb);
+  L.finish(slots, nb, sum);
 }
 
 // Orders the LDS accesses of one wavefront around this point (a wave executes its
@@ -518,7 +561,9 @@ struct StepArgs {
   kf_icp_stats* st;
 };
 
-template <bool COMP>
+// K: pixels per thread at this level's grid (the model samples of all of them are
+// loaded before the solve).
+template <bool COMP, int K>
 __global__ __launch_bounds__(kStepThreads) void icp_step(IcpArgs a, StepArgs S) {
   __shared__ double sum[kTerms + 2], Tin[16], Tnew[16];
   __shared__ kf_posef s_wc, s_cw;
@@ -528,7 +573,11 @@ __global__ __launch_bounds__(kStepThreads) void icp_step(IcpArgs a, StepArgs S) 
     if (blockIdx.x == 0) copy_pose(S.pose_in, S.pose_out);
     return;
   }
-  const PxPair pre = prefetch_models<kStepThreads / kWave>(a, blockIdx.x, gridDim.x);
+  // the previous step's slots first, then the model samples, so that the solve
+  // waits only for its own loads
+  SlotSum<kStepThreads> slots_in;
+  if (!S.first) slots_in.load(S.slots_in, S.nb_in);
+  const PxPre<K> pre = prefetch_models<kStepThreads / kWave, K>(a, blockIdx.x, gridDim.x);
   if (S.first) {
     if (tid < 12) {
       if (tid < 9) s_wc.r[tid] = S.pose_in->wc.r[tid];
@@ -547,7 +596,7 @@ __global__ __launch_bounds__(kStepThreads) void icp_step(IcpArgs a, StepArgs S) 
       }
     }
   } else {
-    sum_slots<kStepThreads>(S.slots_in, S.nb_in, sum);
+    slots_in.finish(S.slots_in, S.nb_in, sum);
     if (tid < 16) Tin[tid] = S.pose_in->T_wc[tid];
     __syncthreads();
     const bool ok = solve_compose(sum, Tin, blockIdx.x == 0 ? S.st : nullptr, S.level_in, Tnew, &s_wc, &s_cw);
@@ -573,13 +622,10 @@ __global__ __launch_bounds__(kStepThreads) void icp_step(IcpArgs a, StepArgs S) 
     }
   }
   __syncthreads();
-  block_partial<COMP, kStepThreads / kWave>(a, s_wc, s_cw, blockIdx.x, gridDim.x,
-                                            S.slots_out + (size_t)blockIdx.x * kSlot, &pre);
+  block_partial<COMP, kStepThreads / kWave, K, true>(a, s_wc, s_cw, blockIdx.x, gridDim.x,
+                                                     S.slots_out + (size_t)blockIdx.x * kSlot, pre);
 }
 
-// End of tracking (one block): solves the last step's slots, writes the output
-// pose and the statistics (iterations = updates applied), copies them to the
-// caller's statistics.  `steps` = 0: no step ran; the guess is the result.
 // Acceptance test of the finished call (IcpGate).  The window's median: lane i
 // ranks value i (ties broken by index) and the lanes of the middle rank(s) publish
 // their values, one pass instead of a serial sort.  Every thread of the block calls it.
@@ -615,6 +661,10 @@ __device__ void gate_decide(const IcpGate& g, const kf_icp_stats* sp, const kf_p
   if (!s_fail) copy_pose(pose, g.accept_out);
 }
 
+// End of tracking (one block): solves the last step's slots, writes the output
+// pose and the statistics (iterations = updates applied), copies them to the
+// caller's statistics, then runs the acceptance test when gated.  `steps` = 0: no
+// step ran; the guess is the result.
 __global__ __launch_bounds__(kSolveThreads) void icp_finish(IcpArgs a, StepArgs S, int steps, kf_pose* out,
                                                             kf_icp_stats* user, IcpGate gate, int gated) {
   __shared__ double sum[kTerms + 2], Tin[16], Tnew[16];
@@ -863,9 +913,11 @@ IcpArgs level_args(const kf_tracker* t, int l, const kf_frame_maps& live, const 
 }
 
 // Tracking steps: 512-thread blocks, 4 / 2 / 1 pixels per thread at levels 0 / 1 / 2+.
+// Pixels per thread of a step's grid at level l (4 / 2 / 1).
+int step_ppt(int l) { return l == 0 ? 4 : (l == 1 ? 2 : 1); }
+
 int step_blocks(const kf_tracker* t, int W, int H, int l) {
-  const int ppt = l == 0 ? 4 : (l == 1 ? 2 : 1);
-  int b = div_up(W * H, kStepThreads * ppt);
+  int b = div_up(W * H, kStepThreads * step_ppt(l));
   return b < t->max_blocks ? b : t->max_blocks;
 }
 
@@ -952,10 +1004,16 @@ kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_fr
       S.slots_out = slots[step & 1];
       S.nb_in = prev_blocks;
       S.level_in = prev_level;
-      if (p.compensated)
-        hipLaunchKernelGGL(icp_step<true>, dim3(blocks), dim3(kStepThreads), 0, s, a, S);
-      else
-        hipLaunchKernelGGL(icp_step<false>, dim3(blocks), dim3(kStepThreads), 0, s, a, S);
+      const int K = step_ppt(l);
+      if (p.compensated) {
+        if (K == 4) hipLaunchKernelGGL((icp_step<true, 4>), dim3(blocks), dim3(kStepThreads), 0, s, a, S);
+        else if (K == 2) hipLaunchKernelGGL((icp_step<true, 2>), dim3(blocks), dim3(kStepThreads), 0, s, a, S);
+        else hipLaunchKernelGGL((icp_step<true, 1>), dim3(blocks), dim3(kStepThreads), 0, s, a, S);
+      } else {
+        if (K == 4) hipLaunchKernelGGL((icp_step<false, 4>), dim3(blocks), dim3(kStepThreads), 0, s, a, S);
+        else if (K == 2) hipLaunchKernelGGL((icp_step<false, 2>), dim3(blocks), dim3(kStepThreads), 0, s, a, S);
+        else hipLaunchKernelGGL((icp_step<false, 1>), dim3(blocks), dim3(kStepThreads), 0, s, a, S);
+      }
       KF_TRY(check_launch("icp_step"));
       S.pose_in = S.pose_out;
       prev_blocks = blocks;

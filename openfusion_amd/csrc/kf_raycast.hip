@@ -63,14 +63,6 @@ __device__ __forceinline__ float lerp(float a, float b, float t) { return a + (b
 template <bool OFF32>
 using voff_t = typename std::conditional<OFF32, unsigned, size_t>::type;
 
-// x returned through an empty asm statement: the compiler cannot see its range, so
-// it neither widens the arithmetic that uses it to 64 bits nor fuses a 24-bit
-// multiply and add into a 64-bit multiply-add.
-__device__ __forceinline__ unsigned opaque_u32(unsigned x) {
-  asm("" : "+v"(x));
-  return x;
-}
-
 template <int FMT, bool OFF32>
 __device__ __forceinline__ voff_t<OFF32> vox_offset(int n, int ix, int iy, int iz) {
   constexpr unsigned kB = FMT == KF_VOXEL_F16 ? 4u : 8u;
