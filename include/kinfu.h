@@ -289,7 +289,9 @@ kf_status kf_tracker_create(const kf_intrinsics* k0, const kf_icp_params* p, kf_
 kf_status kf_tracker_destroy(kf_tracker* t);
 /* Tracks `live` against `model` starting at *d_guess; writes the result to *d_out
  * (may alias d_guess).  d_stats (device, nullable): written with the call's stats;
- * if d_stats->skip is nonzero at execution time, the call leaves *d_out untouched. */
+ * if d_stats->skip is nonzero at execution time, the call leaves *d_out and *d_stats
+ * untouched (its iteration launches still run unless the tracker's parameters ask
+ * for the single-launch form, where a skipped call costs one short launch). */
 kf_status kf_icp_track(kf_tracker* t, const kf_frame_maps* live, const kf_frame_maps* model,
                        const kf_pose* d_guess, kf_pose* d_out, kf_icp_stats* d_stats, kf_stream s);
 

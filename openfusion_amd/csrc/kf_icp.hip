@@ -32,7 +32,9 @@ namespace {
 
 constexpr int kTerms = 28;   // 21 + 6 + 1 float terms
 constexpr int kPad = 32;     // terms padded to a power of two for the wave reduction
-constexpr int kSlot = 64;    // floats per block partial: 28 (hi,lo) + 2 int counts
+constexpr int kSlot = 64;    // floats per block partial: 28 (hi,lo) + 2 int counts + stop marker
+constexpr int kStopAt = 58;  // slot float (int bits): 1 when the block stopped (singular system)
+constexpr int kSums = kTerms + 3;  // slot sums: 28 terms, inliers, candidates, stopped blocks
 constexpr int kThreads = 256;
 constexpr double kPivotRelEps = 1e-9;
 constexpr int kMinInliers = 6;
@@ -320,10 +322,12 @@ __device__ __forceinline__ void block_partial(const IcpArgs& a, const kf_posef& 
     }
     slot[56] = __int_as_float(ni);
     slot[57] = __int_as_float(nc);
+    slot[kStopAt] = __int_as_float(0);
   }
 }
 
-// FP64 sum of nb slots into sum[30] (LDS, valid after the call for every thread).
+// FP64 sum of nb slots into sum[kSums] (LDS, valid after the call for every
+// thread): the 28 terms, inliers, candidates and the number of stopped blocks.
 // Thread (g, j) reads float4 j of the slots of blocks g, g + G, ... (j < 14: terms
 // 2j, 2j + 1 as (hi, lo) pairs; j = 14: the counts) with kFinUnroll loads in
 // flight; the group totals are combined in a fixed order (lane shuffles within a
@@ -350,10 +354,10 @@ struct SlotSum {
 template <int THREADS>
 __device__ __forceinline__ void SlotSum<THREADS>::finish(const float* slots, int nb, double* sum) {
   constexpr int WV = THREADS / kWave;
-  __shared__ double fin[WV][kTerms + 2];
+  __shared__ double fin[WV][kSums];
   const int tid = threadIdx.x;
   const int j = tid & 15, g = tid >> 4;
-  double a0 = 0.0, a1 = 0.0;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   if (j < 15) {
     const float4* base = reinterpret_cast<const float4*>(slots) + j;
     for (int b0 = g; b0 < nb; b0 += G * kFinUnroll) {
@@ -370,6 +374,7 @@ __device__ __forceinline__ void SlotSum<THREADS>::finish(const float* slots, int
         } else {
           a0 = a0 + (double)__float_as_int(v[u].x);
           a1 = a1 + (double)__float_as_int(v[u].y);
+          a2 = a2 + (double)__float_as_int(v[u].z);
         }
       }
     }
@@ -379,12 +384,15 @@ __device__ __forceinline__ void SlotSum<THREADS>::finish(const float* slots, int
   a1 = a1 + __shfl_xor(a1, 16, kWave);
   a0 = a0 + __shfl_xor(a0, 32, kWave);
   a1 = a1 + __shfl_xor(a1, 32, kWave);
+  a2 = a2 + __shfl_xor(a2, 16, kWave);
+  a2 = a2 + __shfl_xor(a2, 32, kWave);
   if ((tid & 63) < 15) {
     fin[tid >> 6][2 * j] = a0;
     fin[tid >> 6][2 * j + 1] = a1;
   }
+  if ((tid & 63) == 14) fin[tid >> 6][kTerms + 2] = a2;
   __syncthreads();
-  if (tid < kTerms + 2) {
+  if (tid < kSums) {
     double t = fin[0][tid];
     for (int w = 1; w < WV; ++w) t = t + fin[w][tid];
     sum[tid] = t;
@@ -537,7 +545,7 @@ __global__ __launch_bounds__(kThreads) void icp_partials(IcpArgs a) {
 
 // Reduce mode, pass 2 (one block): the FP64 sums.
 __global__ __launch_bounds__(kSolveThreads) void icp_sums(IcpArgs a, int nb) {
-  __shared__ double sum[kTerms + 2];
+  __shared__ double sum[kSums];
   sum_slots<kSolveThreads>(a.partials, nb, sum);
   if ((int)threadIdx.x < kTerms + 2) a.sums_out[threadIdx.x] = sum[threadIdx.x];
 }
@@ -557,7 +565,8 @@ struct StepArgs {
   int nb_in;                // number of slots in slots_in
   int level_in;             // level of the previous step
   int first;
-  const kf_icp_stats* user;  // its skip flag skips the whole call (prior fallback not needed)
+  int index;                 // step number in the call (step k solves iteration k - 1)
+  const kf_icp_stats* user;  // its skip flag makes icp_finish leave the outputs untouched
   kf_icp_stats* st;
 };
 
@@ -565,18 +574,20 @@ struct StepArgs {
 // loaded before the solve).
 template <bool COMP, int K>
 __global__ __launch_bounds__(kStepThreads) void icp_step(IcpArgs a, StepArgs S) {
-  __shared__ double sum[kTerms + 2], Tin[16], Tnew[16];
+  __shared__ double sum[kSums], Tin[16], Tnew[16];
   __shared__ kf_posef s_wc, s_cw;
   const int tid = threadIdx.x;
-  if (S.user && S.user->skip) return;  // uniform
-  if (!S.first && a.ctrl->stop) {      // an earlier solve was singular: keep the pose
-    if (blockIdx.x == 0) copy_pose(S.pose_in, S.pose_out);
-    return;
-  }
-  // the previous step's slots first, then the model samples, so that the solve
-  // waits only for its own loads
+  float* slot_out = S.slots_out + (size_t)blockIdx.x * kSlot;
+  // Every load that does not depend on another is issued up front, in the order
+  // they are needed (loads complete in issue order): the previous step's slots
+  // and pose, then the model samples.  No flag is read before them: a singular
+  // solve is passed on through the slots' stop markers.
   SlotSum<kStepThreads> slots_in;
-  if (!S.first) slots_in.load(S.slots_in, S.nb_in);
+  double tin = 0.0;
+  if (!S.first) {
+    slots_in.load(S.slots_in, S.nb_in);
+    tin = S.pose_in->T_wc[tid & 15];
+  }
   const PxPre<K> pre = prefetch_models<kStepThreads / kWave, K>(a, blockIdx.x, gridDim.x);
   if (S.first) {
     if (tid < 12) {
@@ -589,7 +600,6 @@ __global__ __launch_bounds__(kStepThreads) void icp_step(IcpArgs a, StepArgs S) 
     if (blockIdx.x == 0) {
       copy_pose(S.pose_in, S.pose_out);
       if (tid == 32) {
-        a.ctrl->stop = 0;
         a.ctrl->executed = 0;
         kf_icp_stats z = {};
         *S.st = z;
@@ -597,14 +607,15 @@ __global__ __launch_bounds__(kStepThreads) void icp_step(IcpArgs a, StepArgs S) 
     }
   } else {
     slots_in.finish(S.slots_in, S.nb_in, sum);
-    if (tid < 16) Tin[tid] = S.pose_in->T_wc[tid];
+    if (tid < 16) Tin[tid] = tin;
     __syncthreads();
-    const bool ok = solve_compose(sum, Tin, blockIdx.x == 0 ? S.st : nullptr, S.level_in, Tnew, &s_wc, &s_cw);
+    // an earlier solve was singular (a previous block stopped), or this one is:
+    // keep the pose and pass the stop on
+    bool ok = sum[kTerms + 2] == 0.0;  // uniform
+    if (ok) ok = solve_compose(sum, Tin, blockIdx.x == 0 ? S.st : nullptr, S.level_in, Tnew, &s_wc, &s_cw);
     if (!ok) {
-      if (blockIdx.x == 0) {
-        copy_pose(S.pose_in, S.pose_out);
-        if (tid == 32) a.ctrl->stop = 1;
-      }
+      if (blockIdx.x == 0) copy_pose(S.pose_in, S.pose_out);
+      if (tid == 0) slot_out[kStopAt] = __int_as_float(1);
       return;  // uniform
     }
     if (blockIdx.x == 0) {
@@ -617,13 +628,12 @@ __global__ __launch_bounds__(kStepThreads) void icp_step(IcpArgs a, StepArgs S) 
         const int q = tid - 128;
         if (q < 9) P->cw.r[q] = s_cw.r[q]; else P->cw.t[q - 9] = s_cw.t[q - 9];
       } else if (tid == 192) {
-        a.ctrl->executed = a.ctrl->executed + 1;
+        a.ctrl->executed = S.index;  // every earlier solve succeeded too
       }
     }
   }
   __syncthreads();
-  block_partial<COMP, kStepThreads / kWave, K, true>(a, s_wc, s_cw, blockIdx.x, gridDim.x,
-                                                     S.slots_out + (size_t)blockIdx.x * kSlot, pre);
+  block_partial<COMP, kStepThreads / kWave, K, true>(a, s_wc, s_cw, blockIdx.x, gridDim.x, slot_out, pre);
 }
 
 // Acceptance test of the finished call (IcpGate).  The window's median: lane i
@@ -667,7 +677,7 @@ __device__ void gate_decide(const IcpGate& g, const kf_icp_stats* sp, const kf_p
 // step ran; the guess is the result.
 __global__ __launch_bounds__(kSolveThreads) void icp_finish(IcpArgs a, StepArgs S, int steps, kf_pose* out,
                                                             kf_icp_stats* user, IcpGate gate, int gated) {
-  __shared__ double sum[kTerms + 2], Tin[16], Tnew[16];
+  __shared__ double sum[kSums], Tin[16], Tnew[16];
   __shared__ kf_posef s_wc, s_cw;
   const int tid = threadIdx.x;
   if (S.user && S.user->skip) return;  // skipped call: outputs untouched
@@ -676,16 +686,15 @@ __global__ __launch_bounds__(kSolveThreads) void icp_finish(IcpArgs a, StepArgs 
     if (tid == 32) {
       kf_icp_stats z = {};
       *S.st = z;
-      a.ctrl->stop = 0;
       a.ctrl->executed = 0;
     }
-  } else if (a.ctrl->stop) {
-    copy_pose(S.pose_in, out);
   } else {
     sum_slots<kSolveThreads>(S.slots_in, S.nb_in, sum);
     if (tid < 16) Tin[tid] = S.pose_in->T_wc[tid];
     __syncthreads();
-    if (solve_compose(sum, Tin, S.st, S.level_in, Tnew, &s_wc, &s_cw)) {
+    bool ok = sum[kTerms + 2] == 0.0;  // no earlier solve was singular (uniform)
+    if (ok) ok = solve_compose(sum, Tin, S.st, S.level_in, Tnew, &s_wc, &s_cw);
+    if (ok) {
       if (tid < 16) out->T_wc[tid] = Tnew[tid];
       else if (tid >= 64 && tid < 76) {
         const int q = tid - 64;
@@ -694,11 +703,10 @@ __global__ __launch_bounds__(kSolveThreads) void icp_finish(IcpArgs a, StepArgs 
         const int q = tid - 128;
         if (q < 9) out->cw.r[q] = s_cw.r[q]; else out->cw.t[q - 9] = s_cw.t[q - 9];
       } else if (tid == 192) {
-        a.ctrl->executed = a.ctrl->executed + 1;
+        a.ctrl->executed = steps;
       }
     } else {
       copy_pose(S.pose_in, out);
-      if (tid == 32) a.ctrl->stop = 1;
     }
   }
   __syncthreads();
@@ -785,7 +793,7 @@ __device__ __forceinline__ bool grid_barrier(kf_icp_ctrl* ctrl, unsigned nblocks
 
 template <bool COMP>
 __global__ __launch_bounds__(kStepThreads) void icp_chain(ChainArgs C) {
-  __shared__ double sum[kTerms + 2], T[2][16];
+  __shared__ double sum[kSums], T[2][16];
   __shared__ kf_posef s_wc, s_cw;
   const int tid = threadIdx.x;
   const bool lead = blockIdx.x == 0;
@@ -999,6 +1007,7 @@ kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_fr
     const int blocks = step_blocks(t, a.W, a.H, l);
     for (int it = 0; it < p.iterations[l]; ++it, ++step) {
       S.first = step == 0 ? 1 : 0;
+      S.index = step;
       S.pose_out = &t->d_work[step & 1];
       S.slots_in = slots[(step + 1) & 1];
       S.slots_out = slots[step & 1];

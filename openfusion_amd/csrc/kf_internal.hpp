@@ -69,8 +69,8 @@ struct kf_volume {
 // ICP scratch and per-iteration control block (device memory).
 struct kf_icp_ctrl {
   unsigned int arrive;  // single-launch grid barrier: blocks arrived at the current barrier
-  int stop;             // set when a system was singular: later iterations skip
-  int executed;         // iterations that produced an update
+  int unused;
+  int executed;         // multi-launch calls: iterations that produced an update
   unsigned int gen;     // single-launch grid barrier: generation (advanced by the last arrival)
   int broken;           // a barrier wait timed out: single-launch calls report status 2
   int pad[3];

@@ -156,6 +156,7 @@ def test_icp_single_launch_matches_multi_launch(gpu, kf, orc, synth, small):
     live = _gpu_preprocess(kf, k, small.depth[4])
     guess = orc.constant_velocity(small.poses[1], small.poses[2])
     cases = [dict(iters=(10, 5, 4)), dict(iters=(3, 0, 2)), dict(iters=(0, 0, 0)), dict(iters=(4, 3, 3), dist=1e-6)]
+    res_full = None
     for comp in (1, 0):
         for c in cases:
             _say(f"icp single launch compensated={comp} case={c}")
@@ -167,19 +168,26 @@ def test_icp_single_launch_matches_multi_launch(gpu, kf, orc, synth, small):
                     st = icp.Iterate_ICP(live, model, out)
                     res.append((bytes(out.raw()), bytes(st), st.status, st.iterations))
             assert res[1] == res[0] and res[2] == res[0], (res[0][2:], res[1][2:], res[2][2:])
+            if comp == 1 and c["iters"] == (10, 5, 4):
+                res_full = res[0]
             if c.get("dist") == 1e-6:
                 assert res[0][2] == 1 and res[0][3] == 0
             elif sum(c["iters"]) > 0:
                 assert res[0][2] == 0 and res[0][3] == sum(c["iters"])
-    # skip flag: the call does nothing
-    icp = kf.ICP(k, icp_params(single=1))
-    flag = kf.IcpStats()
-    flag.skip = 1
-    icp.stats.upload(np.frombuffer(bytes(flag), np.uint8))
-    out = kf.DevicePose(guess)
-    before = bytes(out.raw())
-    st = icp.Iterate_ICP(live, model, out)
-    assert bytes(out.raw()) == before and bytes(st) == bytes(flag)
+    # skip flag: pose and statistics untouched, in both forms; a call after a
+    # skipped one gives the unskipped result
+    for single in (1, 0):
+        icp = kf.ICP(k, icp_params(single=single))
+        flag = kf.IcpStats()
+        flag.skip = 1
+        icp.stats.upload(np.frombuffer(bytes(flag), np.uint8))
+        out = kf.DevicePose(guess)
+        before = bytes(out.raw())
+        st = icp.Iterate_ICP(live, model, out)
+        assert bytes(out.raw()) == before and bytes(st) == bytes(flag)
+        icp.stats.upload(np.frombuffer(bytes(kf.IcpStats()), np.uint8))
+        st = icp.Iterate_ICP(live, model, out)
+        assert (bytes(out.raw()), bytes(st)) == res_full[:2]
 
 
 def _run_gpu_pipeline(kf, synth, stream, preset, use_graph=True, frames=None):
