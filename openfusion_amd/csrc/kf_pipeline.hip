@@ -197,17 +197,23 @@ kf_status stage_prior_inputs(kf_pipeline* p, const kf_frame_maps& live, hipStrea
 }
 
 // Frame k's part that does not depend on the previous frame's model: preprocessing,
-// the constant-velocity guess and, with a prior, its inference.
+// the constant-velocity guess and, with a prior, its inference and guess.
+// The prior's inputs, inference and guess for the frame whose live maps are `live`.
+kf_status record_prior(kf_pipeline* p, const kf_frame_maps& live, hipStream_t s) {
+  KF_TRY(stage_prior_inputs(p, live, s));
+  char* st = reinterpret_cast<char*>(p->d_prior);  // device addresses of PriorState members
+  KF_TRY(kf_prior_forward(p->prior, p->d_prior_frames, reinterpret_cast<int32_t*>(st + offsetof(PriorState, pairs)),
+                          reinterpret_cast<float*>(st + offsetof(PriorState, prev_twist)), 1,
+                          reinterpret_cast<float*>(st + offsetof(PriorState, out_twist)), s));
+  hipLaunchKernelGGL(prior_guess, dim3(1), dim3(64), 0, s, p->d_prior, p->d_poses);
+  return check_launch("prior_guess");
+}
+
 kf_status record_front(kf_pipeline* p, kf_frame_maps& live, hipStream_t s) {
   const kf_pipeline_desc& d = p->d;
   KF_TRY(launch_preprocess(d.intr, d.pre, p->d_depth, live, s));
   KF_TRY(launch_pose_constant_velocity(&p->d_poses[kPrev2], &p->d_poses[kPrev1], &p->d_poses[kGuess], s));
-  if (!p->prior) return KF_OK;
-  KF_TRY(stage_prior_inputs(p, live, s));
-  char* st = reinterpret_cast<char*>(p->d_prior);  // device addresses of PriorState members
-  return kf_prior_forward(p->prior, p->d_prior_frames, reinterpret_cast<int32_t*>(st + offsetof(PriorState, pairs)),
-                          reinterpret_cast<float*>(st + offsetof(PriorState, prev_twist)), 1,
-                          reinterpret_cast<float*>(st + offsetof(PriorState, out_twist)), s);
+  return p->prior ? record_prior(p, live, s) : KF_OK;
 }
 
 // Tracking of frame k against the model (after record_front): prior guess and
@@ -215,9 +221,7 @@ kf_status record_front(kf_pipeline* p, kf_frame_maps& live, hipStream_t s) {
 kf_status record_track(kf_pipeline* p, kf_frame_maps& live, hipStream_t s) {
   const kf_pipeline_desc& d = p->d;
   if (p->prior) {
-    hipLaunchKernelGGL(prior_guess, dim3(1), dim3(64), 0, s, p->d_prior, p->d_poses);
-    KF_TRY(check_launch("prior_guess"));
-    // the warm-started call ends with the acceptance test: accepted, its pose is the
+    // the warm-started call (from the prior's guess, made by record_front) ends with the acceptance test: accepted, its pose is the
     // frame's and the fallback's skip flag is set
     char* st = reinterpret_cast<char*>(p->d_prior);  // device addresses of PriorState members
     IcpGate g;
@@ -319,13 +323,7 @@ kf_status record_timed_frame(kf_pipeline* p, int k) {
   KF_TRY(launch_preprocess(d.intr, d.pre, p->d_depth, live, s));
   KF_HIP(hipEventRecord(p->ev[1], s));
   KF_TRY(launch_pose_constant_velocity(&p->d_poses[kPrev2], &p->d_poses[kPrev1], &p->d_poses[kGuess], s));
-  if (p->prior) {
-    KF_TRY(stage_prior_inputs(p, live, s));
-    char* st = reinterpret_cast<char*>(p->d_prior);
-    KF_TRY(kf_prior_forward(p->prior, p->d_prior_frames, reinterpret_cast<int32_t*>(st + offsetof(PriorState, pairs)),
-                            reinterpret_cast<float*>(st + offsetof(PriorState, prev_twist)), 1,
-                            reinterpret_cast<float*>(st + offsetof(PriorState, out_twist)), s));
-  }
+  if (p->prior) KF_TRY(record_prior(p, live, s));
   KF_TRY(record_track(p, live, s));
   KF_HIP(hipEventRecord(p->ev[2], s));
   KF_TRY(launch_integrate(p->vol, d.intr, d.integ, live.raylen, &p->d_poses[kCur], &p->d_counters[0], nullptr, s,
