@@ -1,0 +1,80 @@
+// kf_frame_end.hpp — end-of-frame bookkeeping of the frame driver, run on the GPU
+// by the kernel that finishes a frame's tracking (so that it costs no launch of
+// its own): with a learned prior, the accepted residual joins the running window
+// and the network's next motion input is formed; then the frame's pose is logged
+// and the pose history shifts (k-2 <- k-1 <- k).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kf_pose_math.hpp"
+#include "kinfu.h"
+
+namespace kf {
+
+// Pose history slots of the frame driver.
+enum PoseSlot { kCur = 0, kPrev1 = 1, kPrev2 = 2, kGuess = 3, kGuessPrior = 4, kPriorOut = 5, kSlots = 6 };
+
+// Rotation vector of a rotation matrix (row-major 3x3).
+__device__ inline void R_to_rotvec(const double* R, double* w) {
+  const double tr = (R[0] + R[4]) + R[8];
+  const double c = fmin(1.0, fmax(-1.0, 0.5 * (tr - 1.0)));
+  const double th = acos(c);
+  const double v[3] = {R[7] - R[5], R[2] - R[6], R[3] - R[1]};
+  const double s = th < 1e-6 ? 0.5 : th / (2.0 * sin(th));
+  for (int i = 0; i < 3; ++i) w[i] = s * v[i];
+}
+
+struct FrameEnd {
+  kf_pose* poses;  // kSlots history slots
+  double* traj;    // KF_TRAJ_CAPACITY poses, 16 doubles each
+  int* traj_count;
+  // the prior's state (window == nullptr: no prior, or a frame that was not tracked)
+  const kf_icp_stats* warm;   // the warm-started call's statistics
+  const kf_icp_stats* plain;  // the fallback's statistics
+  const int* fell_back;
+  double* window;
+  int* win_head;
+  int* win_count;
+  int window_len;
+  int* fallbacks;
+  int* frames;
+  float* prev_twist;
+};
+
+// Called by every thread of one block (it has a barrier), after the frame's pose is
+// final and visible to the block.
+__device__ inline void frame_end_run(const FrameEnd& f) {
+  const int tid = threadIdx.x;
+  if (f.window && tid == 0) {
+    const double res = *f.fell_back ? f.plain->residual_rms : f.warm->residual_rms;
+    if (res == res && res > 0.0) {
+      f.window[*f.win_head] = res;
+      *f.win_head = (*f.win_head + 1) % f.window_len;
+      if (*f.win_count < f.window_len) *f.win_count += 1;
+    }
+    *f.fallbacks += *f.fell_back;
+    *f.frames += 1;
+    double Ti[16], D[16];
+    rigid_inverse(f.poses[kPrev1].T_wc, Ti);
+    mat4_mul(Ti, f.poses[kCur].T_wc, D);
+    const double R[9] = {D[0], D[1], D[2], D[4], D[5], D[6], D[8], D[9], D[10]};
+    double w[3];
+    R_to_rotvec(R, w);
+    for (int i = 0; i < 3; ++i) {
+      f.prev_twist[i] = (float)(w[i] / KF_PRIOR_ROT_SCALE);
+      f.prev_twist[3 + i] = (float)(D[i * 4 + 3] / KF_PRIOR_TRANS_SCALE);
+    }
+  }
+  if (tid < 16) {
+    const int c = *f.traj_count;
+    if (c < KF_TRAJ_CAPACITY) f.traj[(size_t)c * 16 + tid] = f.poses[kCur].T_wc[tid];
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  *f.traj_count = *f.traj_count + 1;
+  f.poses[kPrev2] = f.poses[kPrev1];
+  f.poses[kPrev1] = f.poses[kCur];
+}
+
+}  // namespace kf

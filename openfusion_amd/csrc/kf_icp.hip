@@ -24,6 +24,7 @@
 // block of an iteration repeats that solve for itself, so the pose never makes a
 // round trip through the host and an iteration is a single launch.
 #include "kf_device.hpp"
+#include "kf_frame_end.hpp"
 #include "kf_internal.hpp"
 #include "kf_pose_math.hpp"
 
@@ -676,11 +677,15 @@ __device__ void gate_decide(const IcpGate& g, const kf_icp_stats* sp, const kf_p
 // caller's statistics, then runs the acceptance test when gated.  `steps` = 0: no
 // step ran; the guess is the result.
 __global__ __launch_bounds__(kSolveThreads) void icp_finish(IcpArgs a, StepArgs S, int steps, kf_pose* out,
-                                                            kf_icp_stats* user, IcpGate gate, int gated) {
+                                                            kf_icp_stats* user, IcpGate gate, int gated,
+                                                            FrameEnd fe, int fe_on) {
   __shared__ double sum[kSums], Tin[16], Tnew[16];
   __shared__ kf_posef s_wc, s_cw;
   const int tid = threadIdx.x;
-  if (S.user && S.user->skip) return;  // skipped call: outputs untouched
+  if (S.user && S.user->skip) {  // skipped call: outputs untouched (uniform)
+    if (fe_on) frame_end_run(fe);
+    return;
+  }
   if (steps == 0) {
     copy_pose(S.pose_in, out);
     if (tid == 32) {
@@ -722,6 +727,10 @@ __global__ __launch_bounds__(kSolveThreads) void icp_finish(IcpArgs a, StepArgs 
     __syncthreads();
     gate_decide(gate, S.st, out);
   }
+  if (fe_on) {  // uniform
+    __syncthreads();
+    frame_end_run(fe);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -754,6 +763,8 @@ struct ChainArgs {
   kf_icp_stats* user;  // skip flag; receives a copy of the statistics
   kf_icp_stats* st;
   kf_icp_ctrl* ctrl;
+  FrameEnd fe;         // run by block 0 at the end when fe_on
+  int fe_on;
 };
 
 // Grid barrier: the last block to arrive resets the arrival count and advances the
@@ -797,7 +808,10 @@ __global__ __launch_bounds__(kStepThreads) void icp_chain(ChainArgs C) {
   __shared__ kf_posef s_wc, s_cw;
   const int tid = threadIdx.x;
   const bool lead = blockIdx.x == 0;
-  if (C.user && C.user->skip) return;  // uniform
+  if (C.user && C.user->skip) {  // uniform
+    if (lead && C.fe_on) frame_end_run(C.fe);
+    return;
+  }
   kf_icp_stats* st = C.st;
   if (C.ctrl->broken) {  // uniform: written by an earlier launch
     if (lead && tid == 0) {
@@ -807,6 +821,10 @@ __global__ __launch_bounds__(kStepThreads) void icp_chain(ChainArgs C) {
       if (C.user && C.user != st) *C.user = z;
     }
     if (lead) copy_pose(C.guess, C.out);
+    if (lead && C.fe_on) {
+      __syncthreads();
+      frame_end_run(C.fe);
+    }
     return;
   }
   if (tid < 16) T[0][tid] = C.guess->T_wc[tid];
@@ -879,6 +897,10 @@ __global__ __launch_bounds__(kStepThreads) void icp_chain(ChainArgs C) {
       c.skip = 0;
       *C.user = c;
     }
+  }
+  if (C.fe_on) {
+    __syncthreads();
+    frame_end_run(C.fe);
   }
 }
 
@@ -954,7 +976,7 @@ kf_status check_maps(const kf_tracker* t, const kf_frame_maps& live, const kf_fr
 
 kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_frame_maps& model,
                            const kf_pose* d_guess, kf_pose* d_out, kf_icp_stats* d_stats,
-                           hipStream_t s, bool single_launch, const IcpGate* gate) {
+                           hipStream_t s, bool single_launch, const IcpGate* gate, const FrameEnd* frame_end) {
   KF_CHECK_ARG(d_guess && d_out, "guess/out pose is NULL");
   const kf_icp_params& p = t->params;
   KF_TRY(check_maps(t, live, model, p.levels));
@@ -990,6 +1012,8 @@ kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_fr
     C.user = d_stats;
     C.st = st;
     C.ctrl = t->d_ctrl;
+    C.fe = frame_end ? *frame_end : FrameEnd{};
+    C.fe_on = frame_end ? 1 : 0;
     if (p.compensated)
       hipLaunchKernelGGL(icp_chain<true>, dim3(grid), dim3(kStepThreads), 0, s, C);
     else
@@ -1033,8 +1057,9 @@ kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_fr
   S.nb_in = prev_blocks;
   S.level_in = prev_level;
   const IcpGate g = gate ? *gate : IcpGate{};
+  const FrameEnd fe = frame_end ? *frame_end : FrameEnd{};
   hipLaunchKernelGGL(icp_finish, dim3(1), dim3(kSolveThreads), 0, s, a, S, step, d_out, d_stats, g,
-                     gate ? 1 : 0);
+                     gate ? 1 : 0, fe, frame_end ? 1 : 0);
   return check_launch("icp_finish");
 }
 

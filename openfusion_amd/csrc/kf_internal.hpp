@@ -135,12 +135,16 @@ struct IcpGate {
   float ratio, min_frac;
   int window_min;
 };
+struct FrameEnd;  // kf_frame_end.hpp
 // single_launch: the whole call as one launch (also when the tracker's parameters
 // do not ask for it); used where most calls are skipped by the statistics' flag.
 // gate (multi-launch calls only): evaluated by the call's last kernel.
+// frame_end: the frame driver's end-of-frame bookkeeping, run by the call's last
+// kernel once the pose is final (also when the call is skipped).
 kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_frame_maps& model,
                            const kf_pose* d_guess, kf_pose* d_out, kf_icp_stats* d_stats,
-                           hipStream_t s, bool single_launch = false, const IcpGate* gate = nullptr);
+                           hipStream_t s, bool single_launch = false, const IcpGate* gate = nullptr,
+                           const FrameEnd* frame_end = nullptr);
 kf_status launch_icp_reduce(kf_tracker* t, int level, const kf_frame_maps& live,
                             const kf_frame_maps& model, const kf_pose* d_est, double* d_sums,
                             uint8_t* d_mask, hipStream_t s);

@@ -18,12 +18,19 @@
 #include <stddef.h>
 #include <string.h>
 
+#include "kf_frame_end.hpp"
 #include "kf_internal.hpp"
 #include "kf_pose_math.hpp"
 
 namespace {
 
-enum PoseSlot { kCur = 0, kPrev1 = 1, kPrev2 = 2, kGuess = 3, kGuessPrior = 4, kPriorOut = 5, kSlots = 6 };
+using kf::kCur;
+using kf::kPrev1;
+using kf::kPrev2;
+using kf::kGuess;
+using kf::kGuessPrior;
+using kf::kPriorOut;
+using kf::kSlots;
 
 constexpr int kCounters = 6;
 constexpr int kWindow = 31;      // accepted residuals kept for the running median
@@ -110,15 +117,6 @@ __device__ void rotvec_to_R(const double* w, double* R) {
     }
 }
 
-__device__ void R_to_rotvec(const double* R, double* w) {
-  const double tr = (R[0] + R[4]) + R[8];
-  const double c = fmin(1.0, fmax(-1.0, 0.5 * (tr - 1.0)));
-  const double th = acos(c);
-  const double v[3] = {R[7] - R[5], R[2] - R[6], R[3] - R[1]};
-  const double s = th < 1e-6 ? 0.5 : th / (2.0 * sin(th));
-  for (int i = 0; i < 3; ++i) w[i] = s * v[i];
-}
-
 // guess = T_{k-1} * [exp(w) | t] from the network's scaled twist; the
 // constant-velocity guess if the output is not finite.
 __global__ void prior_guess(PriorState* st, kf_pose* poses) {
@@ -150,41 +148,9 @@ __global__ void prior_guess(PriorState* st, kf_pose* poses) {
   poses[kGuessPrior] = np;
 }
 
-// Frame end.  With a prior: record the accepted residual and the network's next
-// motion input (scaled twist of T_{k-1}^-1 T_k).  Then log the frame's pose and
-// shift the history (k-2 <- k-1 <- k).
-__global__ void frame_end(PriorState* st, const kf_icp_stats* sp, const kf_icp_stats* plain, kf_pose* poses,
-                          double* traj, int* count) {
-  if (st && threadIdx.x == 0) {
-    const double res = st->fell_back ? plain->residual_rms : sp->residual_rms;
-    if (res == res && res > 0.0) {
-      st->window[st->win_head] = res;
-      st->win_head = (st->win_head + 1) % kWindow;
-      if (st->win_count < kWindow) st->win_count += 1;
-    }
-    st->fallbacks += st->fell_back;
-    st->frames += 1;
-    double Ti[16], D[16];
-    rigid_inverse(poses[kPrev1].T_wc, Ti);
-    mat4_mul(Ti, poses[kCur].T_wc, D);
-    const double R[9] = {D[0], D[1], D[2], D[4], D[5], D[6], D[8], D[9], D[10]};
-    double w[3];
-    R_to_rotvec(R, w);
-    for (int i = 0; i < 3; ++i) {
-      st->prev_twist[i] = (float)(w[i] / KF_PRIOR_ROT_SCALE);
-      st->prev_twist[3 + i] = (float)(D[i * 4 + 3] / KF_PRIOR_TRANS_SCALE);
-    }
-  }
-  if (threadIdx.x < 16) {
-    const int c = *count;
-    if (c < KF_TRAJ_CAPACITY) traj[(size_t)c * 16 + threadIdx.x] = poses[kCur].T_wc[threadIdx.x];
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  *count = *count + 1;
-  poses[kPrev2] = poses[kPrev1];
-  poses[kPrev1] = poses[kCur];
-}
+// End of a frame that the tracking kernels did not end (the first frame, and every
+// frame in stage-timing mode).
+__global__ void frame_end(FrameEnd f) { frame_end_run(f); }
 
 // Live maps of frame f (the two sets alternate).
 kf_frame_maps& live_of(kf_pipeline* p, int f) { return p->live2[f & 1]; }
@@ -216,9 +182,33 @@ kf_status record_front(kf_pipeline* p, kf_frame_maps& live, hipStream_t s) {
   return p->prior ? record_prior(p, live, s) : KF_OK;
 }
 
-// Tracking of frame k against the model (after record_front): prior guess and
-// warm-started ICP with the fallback, or plain ICP; then the pose log and history.
-kf_status record_track(kf_pipeline* p, kf_frame_maps& live, hipStream_t s) {
+// The frame-end bookkeeping's arguments; tracked: the frame was tracked (not the
+// first one), so the prior's state is updated too.
+FrameEnd frame_end_args(kf_pipeline* p, bool tracked) {
+  FrameEnd f = {};
+  f.poses = p->d_poses;
+  f.traj = p->d_traj;
+  f.traj_count = p->d_traj_count;
+  if (p->prior && tracked) {
+    char* st = reinterpret_cast<char*>(p->d_prior);  // device addresses of PriorState members
+    f.warm = &p->d_stats[1];
+    f.plain = &p->d_stats[0];
+    f.fell_back = reinterpret_cast<const int*>(st + offsetof(PriorState, fell_back));
+    f.window = reinterpret_cast<double*>(st + offsetof(PriorState, window));
+    f.win_head = reinterpret_cast<int*>(st + offsetof(PriorState, win_head));
+    f.win_count = reinterpret_cast<int*>(st + offsetof(PriorState, win_count));
+    f.window_len = kWindow;
+    f.fallbacks = reinterpret_cast<int*>(st + offsetof(PriorState, fallbacks));
+    f.frames = reinterpret_cast<int*>(st + offsetof(PriorState, frames));
+    f.prev_twist = reinterpret_cast<float*>(st + offsetof(PriorState, prev_twist));
+  }
+  return f;
+}
+
+// Tracking of frame k against the model (after record_front): warm-started ICP
+// from the prior's guess with the fallback, or plain ICP.  fe: the frame-end
+// bookkeeping, run by the last tracking kernel (nullptr: the caller launches it).
+kf_status record_track(kf_pipeline* p, kf_frame_maps& live, hipStream_t s, const FrameEnd* fe = nullptr) {
   const kf_pipeline_desc& d = p->d;
   if (p->prior) {
     // the warm-started call (from the prior's guess, made by record_front) ends with the acceptance test: accepted, its pose is the
@@ -239,18 +229,17 @@ kf_status record_track(kf_pipeline* p, kf_frame_maps& live, hipStream_t s) {
     // the fallback: plain ICP from the constant-velocity guess, skipped in most
     // frames, hence one launch for the whole call
     KF_TRY(launch_icp_track(p->tracker, live, p->model, &p->d_poses[kGuess], &p->d_poses[kCur], &p->d_stats[0], s,
-                            true));
+                            true, nullptr, fe));
   } else {
-    KF_TRY(launch_icp_track(p->tracker, live, p->model, &p->d_poses[kGuess], &p->d_poses[kCur], &p->d_stats[0], s));
+    KF_TRY(launch_icp_track(p->tracker, live, p->model, &p->d_poses[kGuess], &p->d_poses[kCur], &p->d_stats[0], s,
+                            false, nullptr, fe));
   }
   return KF_OK;
 }
 
-// End of a frame; tracked: the frame was tracked (not the first one).
+// End of a frame as a kernel of its own; tracked: as frame_end_args.
 kf_status record_frame_end(kf_pipeline* p, hipStream_t s, bool tracked) {
-  hipLaunchKernelGGL(frame_end, dim3(1), dim3(64), 0, s, p->prior && tracked ? p->d_prior : nullptr,
-                     (const kf_icp_stats*)&p->d_stats[1], (const kf_icp_stats*)&p->d_stats[0], p->d_poses,
-                     p->d_traj, p->d_traj_count);
+  hipLaunchKernelGGL(frame_end, dim3(1), dim3(64), 0, s, frame_end_args(p, tracked));
   return check_launch("frame_end");
 }
 
@@ -290,12 +279,12 @@ kf_status record_depth_copy(kf_pipeline* p, const uint16_t* src, hipStream_t st,
 kf_status record_call(kf_pipeline* p, int k, bool finish, const uint16_t* src = nullptr,
                       hipGraphNode_t* copy_node = nullptr) {
   hipStream_t s = p->stream;
+  const FrameEnd fe = frame_end_args(p, true);
   if (!p->prior) {
     if (src) KF_TRY(record_depth_copy(p, src, s, copy_node));
     if (finish) KF_TRY(record_finish(p, live_of(p, k - 1), s));
     KF_TRY(record_front(p, live_of(p, k), s));
-    KF_TRY(record_track(p, live_of(p, k), s));
-    return record_frame_end(p, s, true);
+    return record_track(p, live_of(p, k), s, &fe);
   }
   // The pipeline stream's part is recorded first: graph replay starts the
   // branches' nodes in recording order, and that part is the longer one (measured
@@ -307,8 +296,7 @@ kf_status record_call(kf_pipeline* p, int k, bool finish, const uint16_t* src = 
   KF_TRY(record_front(p, live_of(p, k), p->side));
   KF_HIP(hipEventRecord(p->join_ev, p->side));
   KF_HIP(hipStreamWaitEvent(s, p->join_ev, 0));
-  KF_TRY(record_track(p, live_of(p, k), s));
-  return record_frame_end(p, s, true);
+  return record_track(p, live_of(p, k), s, &fe);
 }
 
 // One frame in stage-timing mode (eager, one stream, nothing deferred): events
