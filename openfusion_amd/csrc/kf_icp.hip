@@ -568,6 +568,7 @@ struct StepArgs {
   int first;
   int index;                 // step number in the call (step k solves iteration k - 1)
   const kf_icp_stats* user;  // its skip flag makes icp_finish leave the outputs untouched
+                             // (the tracker's own statistics when the caller passes none)
   kf_icp_stats* st;
 };
 
@@ -637,15 +638,17 @@ __global__ __launch_bounds__(kStepThreads) void icp_step(IcpArgs a, StepArgs S) 
   block_partial<COMP, kStepThreads / kWave, K, true>(a, s_wc, s_cw, blockIdx.x, gridDim.x, slot_out, pre);
 }
 
-// Acceptance test of the finished call (IcpGate).  The window's median: lane i
-// ranks value i (ties broken by index) and the lanes of the middle rank(s) publish
-// their values, one pass instead of a serial sort.  Every thread of the block calls it.
-__device__ void gate_decide(const IcpGate& g, const kf_icp_stats* sp, const kf_pose* pose) {
+// Acceptance test of the finished call (IcpGate), from the window count and thread
+// i's window entry (wv = window[i] for i < window_len), loaded by the caller at its
+// start.  The window's median: lane i ranks value i (ties broken by index) and the
+// lanes of the middle rank(s) publish their values, one pass instead of a serial
+// sort.  Every thread of the block calls it.
+__device__ void gate_decide(const IcpGate& g, int count, double wv, const kf_icp_stats* sp, const kf_pose* pose) {
   __shared__ double v[kGateMaxWindow], mid[2];
   __shared__ int s_fail;
   const int i = threadIdx.x;
-  const int n = min(*g.count, kGateMaxWindow);
-  if (i < n) v[i] = g.window[i];
+  const int n = min(count, g.window_len);
+  if (i < n) v[i] = wv;
   __syncthreads();
   if (i < n) {
     const double x = v[i];
@@ -682,24 +685,39 @@ __global__ __launch_bounds__(kSolveThreads) void icp_finish(IcpArgs a, StepArgs 
   __shared__ double sum[kSums], Tin[16], Tnew[16];
   __shared__ kf_posef s_wc, s_cw;
   const int tid = threadIdx.x;
-  if (S.user && S.user->skip) {  // skipped call: outputs untouched (uniform)
-    if (fe_on) frame_end_run(fe);
-    return;
+  // Every independent load first (loads complete in issue order): the skip flag, the
+  // last step's slots and pose, the acceptance test's window.  A skipped call
+  // (uniform) computes like any other into the tracker's own statistics and leaves
+  // the caller's pose and statistics untouched.
+  const int skip = S.user->skip;
+  SlotSum<kSolveThreads> slots_in;
+  double tin = 0.0;
+  if (steps > 0) {
+    slots_in.load(S.slots_in, S.nb_in);
+    tin = S.pose_in->T_wc[tid & 15];
+  }
+  int gcount = 0;
+  double gwv = 0.0;
+  if (gated) {
+    gcount = *gate.count;
+    gwv = gate.window[min(tid, gate.window_len - 1)];
   }
   if (steps == 0) {
-    copy_pose(S.pose_in, out);
+    if (!skip) copy_pose(S.pose_in, out);
     if (tid == 32) {
       kf_icp_stats z = {};
       *S.st = z;
       a.ctrl->executed = 0;
     }
   } else {
-    sum_slots<kSolveThreads>(S.slots_in, S.nb_in, sum);
-    if (tid < 16) Tin[tid] = S.pose_in->T_wc[tid];
+    slots_in.finish(S.slots_in, S.nb_in, sum);
+    if (tid < 16) Tin[tid] = tin;
     __syncthreads();
     bool ok = sum[kTerms + 2] == 0.0;  // no earlier solve was singular (uniform)
     if (ok) ok = solve_compose(sum, Tin, S.st, S.level_in, Tnew, &s_wc, &s_cw);
-    if (ok) {
+    if (skip) {
+      if (ok && tid == 192) a.ctrl->executed = steps;
+    } else if (ok) {
       if (tid < 16) out->T_wc[tid] = Tnew[tid];
       else if (tid >= 64 && tid < 76) {
         const int q = tid - 64;
@@ -717,15 +735,15 @@ __global__ __launch_bounds__(kSolveThreads) void icp_finish(IcpArgs a, StepArgs 
   __syncthreads();
   if (tid == 0) {
     S.st->iterations = a.ctrl->executed;
-    if (user && user != S.st) {
+    if (!skip && user && user != S.st) {
       kf_icp_stats c = *S.st;
       c.skip = 0;
       *user = c;
     }
   }
-  if (gated) {  // uniform
+  if (gated && !skip) {  // uniform
     __syncthreads();
-    gate_decide(gate, S.st, out);
+    gate_decide(gate, gcount, gwv, S.st, out);
   }
   if (fe_on) {  // uniform
     __syncthreads();
@@ -1021,7 +1039,7 @@ kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_fr
     return check_launch("icp_chain");
   }
   StepArgs S = {};
-  S.user = d_stats;
+  S.user = d_stats ? d_stats : st;  // the tracker's own statistics never carry the skip flag
   S.st = st;
   S.pose_in = d_guess;
   int step = 0, prev_blocks = 0, prev_level = 0;

@@ -126,8 +126,9 @@ kf_status launch_raycast(const kf_volume* v, const kf_intrinsics& k0, const kf_r
 // failed: fallback->skip = 0.  *used <- 1, *fell_back <- failed.
 constexpr int kGateMaxWindow = 64;
 struct IcpGate {
-  const double* window;    // recent accepted residuals
-  const int32_t* count;    // how many of them are valid (<= kGateMaxWindow)
+  const double* window;    // recent accepted residuals (window_len entries)
+  const int32_t* count;    // how many of them are valid
+  int window_len;          // <= kGateMaxWindow
   kf_icp_stats* fallback;  // skip flag of the fallback call
   kf_pose* accept_out;
   int32_t* used;

@@ -217,6 +217,7 @@ kf_status record_track(kf_pipeline* p, kf_frame_maps& live, hipStream_t s, const
     IcpGate g;
     g.window = reinterpret_cast<const double*>(st + offsetof(PriorState, window));
     g.count = reinterpret_cast<const int32_t*>(st + offsetof(PriorState, win_count));
+    g.window_len = kWindow;
     g.fallback = &p->d_stats[0];
     g.accept_out = &p->d_poses[kCur];
     g.used = reinterpret_cast<int32_t*>(st + offsetof(PriorState, used_prior));
