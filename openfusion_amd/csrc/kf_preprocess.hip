@@ -38,24 +38,36 @@ struct PreArgs {
 // code difference below kRangeLut), each entry evaluated with exp_neg.
 constexpr int kRangeLut = 512;
 
+// R: the window half-size (a.radius), fixed per instance so that the window loop
+// unrolls.  Pixels outside the image are stored as code 0 in the tile, so the
+// window needs no bounds test of its own: such taps are skipped like invalid ones.
+template <int R>
 __global__ __launch_bounds__(256) void depth_to_l0(PreArgs a) {
   __shared__ int tile[kHalo * kHalo];  // depth codes, 0 = invalid
   __shared__ float ws[(2 * kMaxRadius + 1) * (2 * kMaxRadius + 1)];
   __shared__ float wr[kRangeLut];
-  const int R = a.radius;
   const int span = kTile + 2 * R;
   const int bx = blockIdx.x * kTile, by = blockIdx.y * kTile;
   const int tid = threadIdx.y * kTile + threadIdx.x;
-  for (int i = tid; i < span * span; i += kTile * kTile) {
-    int ty = i / span, tx = i - ty * span;
-    int gx = bx + tx - R, gy = by + ty - R;
-    int u = 0;
-    if (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H) {
-      u = a.depth[gy * a.W + gx];
-      const float d = (float)u * a.scale;
-      if (!(d >= a.dmin && d <= a.dmax)) u = 0;
+  // the tile and its halo, every load of a thread in flight at once (out-of-image
+  // entries read a clamped pixel and store 0)
+  constexpr int kPer = ((kTile + 2 * R) * (kTile + 2 * R) + kTile * kTile - 1) / (kTile * kTile);
+  int uv[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int i = min(tid + j * kTile * kTile, span * span - 1);
+    const int ty = i / span, tx = i - ty * span;
+    const int gx = bx + tx - R, gy = by + ty - R;
+    const int u = a.depth[min(max(gy, 0), a.H - 1) * a.W + min(max(gx, 0), a.W - 1)];
+    uv[j] = (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H) ? u : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int i = tid + j * kTile * kTile;
+    if (i < span * span) {
+      const float d = (float)uv[j] * a.scale;
+      tile[i] = (d >= a.dmin && d <= a.dmax) ? uv[j] : 0;
     }
-    tile[ty * span + tx] = u;
   }
   const int nsp = (2 * R + 1) * (2 * R + 1);
   if (tid < nsp) {
@@ -83,12 +95,10 @@ __global__ __launch_bounds__(256) void depth_to_l0(PreArgs a) {
   a.raylen[idx] = dc * lam;
 
   float sw = 0.0f, swd = 0.0f;
+#pragma unroll
   for (int dy = -R; dy <= R; ++dy) {
-    const int qy = y + dy;
-    if (qy < 0 || qy >= a.H) continue;
+#pragma unroll
     for (int dx = -R; dx <= R; ++dx) {
-      const int qx = x + dx;
-      if (qx < 0 || qx >= a.W) continue;
       const int uq = tile[(threadIdx.y + R + dy) * span + threadIdx.x + R + dx];
       if (uq == 0) continue;
       const int du = uq > uc ? uq - uc : uc - uq;
@@ -223,7 +233,13 @@ kf_status launch_preprocess(const kf_intrinsics& k0, const kf_preprocess_params&
   a.inv2sr = 0.5f / (p.sigma_range_m * p.sigma_range_m);
   dim3 blk(kTile, kTile);
   dim3 grd(div_up(k0.width, kTile), div_up(k0.height, kTile));
-  hipLaunchKernelGGL(depth_to_l0, grd, blk, 0, s, a);
+  switch (a.radius) {
+    case 0: hipLaunchKernelGGL(depth_to_l0<0>, grd, blk, 0, s, a); break;
+    case 1: hipLaunchKernelGGL(depth_to_l0<1>, grd, blk, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(depth_to_l0<2>, grd, blk, 0, s, a); break;
+    case 3: hipLaunchKernelGGL(depth_to_l0<3>, grd, blk, 0, s, a); break;
+    default: hipLaunchKernelGGL(depth_to_l0<4>, grd, blk, 0, s, a); break;
+  }
   KF_TRY(check_launch("depth_to_l0"));
 
   for (int l = 1; l < live.levels; ++l) {
