@@ -54,7 +54,10 @@ __device__ __forceinline__ float vox_at(const char* p) {
   return *reinterpret_cast<const float*>(p);
 }
 
-__device__ __forceinline__ float lerp(float a, float b, float t) { return a + (b - a) * t; }
+// Interpolations, grid coordinates and points along a ray are fused multiply-adds
+// (one rounding each; the oracle computes them the same way).
+__device__ __forceinline__ float lerp(float a, float b, float t) { return fmaf(b - a, t, a); }
+__device__ __forceinline__ f3 ray_at(f3 o, f3 d, float t) { return f3{fmaf(t, d.x, o.x), fmaf(t, d.y, o.y), fmaf(t, d.z, o.z)}; }
 
 // Byte offset of voxel (ix, iy, iz) in the x-major volume.  OFF32: the volume's
 // byte size fits in 32 bits, so the offset is one 32-bit register (24-bit
@@ -117,9 +120,9 @@ __device__ __forceinline__ int sample_points(const RayArgs& a, const f3* pts, bo
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     const f3 p = pts[j];
-    gx[j] = (p.x - a.ox) * a.inv_h - 0.5f;
-    gy[j] = (p.y - a.oy) * a.inv_h - 0.5f;
-    gz[j] = (p.z - a.oz) * a.inv_h - 0.5f;
+    gx[j] = fmaf(p.x - a.ox, a.inv_h, -0.5f);
+    gy[j] = fmaf(p.y - a.oy, a.inv_h, -0.5f);
+    gz[j] = fmaf(p.z - a.oz, a.inv_h, -0.5f);
     // positions are finite (finite pose, t inside the slab interval)
     ok[j] = fminf(fminf(gx[j], gy[j]), gz[j]) >= 0.0f && fmaxf(fmaxf(gx[j], gy[j]), gz[j]) < lim;
     ix[j] = ok[j] ? (int)gx[j] : 0;
@@ -192,7 +195,7 @@ __device__ __forceinline__ int sample_batch(const RayArgs& a, f3 o, f3 dir, cons
                                             float* fk, int* dclear) {
   f3 pts[K];
 #pragma unroll
-  for (int j = 0; j < K; ++j) pts[j] = f3{o.x + tk[j] * dir.x, o.y + tk[j] * dir.y, o.z + tk[j] * dir.z};
+  for (int j = 0; j < K; ++j) pts[j] = ray_at(o, dir, tk[j]);
   return sample_points<FMT, OFF32, K>(a, pts, ok, fk, dclear);
 }
 
@@ -246,7 +249,7 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
   // refinement and trilinear central-difference normal.
   auto hit_at = [&](float t, float st, float fp, float fc) {
     const float ts = t + st * (fp / (fp - fc));
-    const f3 ph = f3{o.x + ts * dir.x, o.y + ts * dir.y, o.z + ts * dir.z};
+    const f3 ph = ray_at(o, dir, ts);
     // the six central-difference samples, fetched as two batches of three
     const f3 q[6] = {f3{ph.x + a.h, ph.y, ph.z}, f3{ph.x - a.h, ph.y, ph.z}, f3{ph.x, ph.y + a.h, ph.z},
                      f3{ph.x, ph.y - a.h, ph.z}, f3{ph.x, ph.y, ph.z + a.h}, f3{ph.x, ph.y, ph.z - a.h}};
@@ -278,7 +281,7 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
     // measured slower.
     constexpr int kSpec = 2;
     float t = tnear, fp;
-    bool ok = sample<FMT, OFF32>(a, f3{o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z}, fp) && fp >= 0.0f;
+    bool ok = sample<FMT, OFF32>(a, ray_at(o, dir, t), fp) && fp >= 0.0f;
     int s = 0;
     // Free-space skip.  From a far-mode position t (sample 1) whose base voxel lies
     // in a brick at distance D >= 1 from the nearest set brick, every brick within
@@ -290,9 +293,10 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
     // intermediate one is too.  The step count, the position and the value are those
     // of the step-by-step march.
     auto skip_free = [&](float& t_, int& s_, int D) {
-      const float gx = ((o.x + t_ * dir.x) - a.ox) * a.inv_h - 0.5f;
-      const float gy = ((o.y + t_ * dir.y) - a.oy) * a.inv_h - 0.5f;
-      const float gz = ((o.z + t_ * dir.z) - a.oz) * a.inv_h - 0.5f;
+      const f3 pt = ray_at(o, dir, t_);
+      const float gx = fmaf(pt.x - a.ox, a.inv_h, -0.5f);
+      const float gy = fmaf(pt.y - a.oy, a.inv_h, -0.5f);
+      const float gz = fmaf(pt.z - a.oz, a.inv_h, -0.5f);
       const int bx = (int)gx >> 3, by = (int)gy >> 3, bz = (int)gz >> 3, r = D - 1;
       const float lim = (float)(a.n - 1);
       const float lx = (float)(8 * max(bx - r, 0)), hx = fminf((float)(8 * (bx + r + 1)), lim);
@@ -313,9 +317,10 @@ __global__ __launch_bounds__(256) void raycast(RayArgs a) {
       float tt = t_;
 #pragma unroll 4
       for (int j = 0; j < k; ++j) tt = tt + a.step_far_m;
-      const float ex = ((o.x + tt * dir.x) - a.ox) * a.inv_h - 0.5f;
-      const float ey = ((o.y + tt * dir.y) - a.oy) * a.inv_h - 0.5f;
-      const float ez = ((o.z + tt * dir.z) - a.oz) * a.inv_h - 0.5f;
+      const f3 pe = ray_at(o, dir, tt);
+      const float ex = fmaf(pe.x - a.ox, a.inv_h, -0.5f);
+      const float ey = fmaf(pe.y - a.oy, a.inv_h, -0.5f);
+      const float ez = fmaf(pe.z - a.oz, a.inv_h, -0.5f);
       if (tt <= tfar && ex >= lx && ex < hx && ey >= ly && ey < hy && ez >= lz && ez < hz) {
         t_ = tt;
         s_ += k;

@@ -18,6 +18,8 @@
 //   * Real = double: the FP64 oracle (FP64 maps, FP64 volume, FP64 ICP) used to
 //     report the error of the FP32/FP16 pipelines.
 #include <math.h>
+
+#include <cmath>
 #include <omp.h>
 #include <stdint.h>
 #include <string.h>
@@ -518,17 +520,19 @@ struct VolView {
     if (d->format == KF_VOXEL_F32) return (R)((const float*)vox)[2 * i];
     return (R)((const double*)vox)[2 * i];
   }
+  // Grid coordinates and interpolations as fused multiply-adds (single rounding),
+  // like the GPU raycaster.
   bool sample(V3<R> p, R& out) const {
-    const R gx = (p.x - ox) * inv_h - R(0.5);
-    const R gy = (p.y - oy) * inv_h - R(0.5);
-    const R gz = (p.z - oz) * inv_h - R(0.5);
+    const R gx = std::fma(p.x - ox, inv_h, R(-0.5));
+    const R gy = std::fma(p.y - oy, inv_h, R(-0.5));
+    const R gz = std::fma(p.z - oz, inv_h, R(-0.5));
     const R lim = (R)(d->n - 1);
     if (!(gx >= R(0) && gx < lim && gy >= R(0) && gy < lim && gz >= R(0) && gz < lim)) return false;
     const int ix = (int)gx, iy = (int)gy, iz = (int)gz;
     const R tx = gx - (R)ix, ty = gy - (R)iy, tz = gz - (R)iz;
     const size_t n = (size_t)d->n;
     const size_t b = ((size_t)ix * n + (size_t)iy) * n + (size_t)iz, sy = n, sx = n * n;
-    auto lerp = [](R a, R c, R t) { return a + (c - a) * t; };
+    auto lerp = [](R a, R c, R t) { return std::fma(c - a, t, a); };
     const R c00 = lerp(f(b), f(b + 1), tz), c01 = lerp(f(b + sy), f(b + sy + 1), tz);
     const R c10 = lerp(f(b + sx), f(b + sx + 1), tz), c11 = lerp(f(b + sx + sy), f(b + sx + sy + 1), tz);
     const R c0 = lerp(c00, c01, ty), c1 = lerp(c10, c11, ty);
@@ -602,19 +606,21 @@ void raycast(const VolDesc& d, const void* vox, const kf_intrinsics& k0, const k
       V3<R> vert = {nanv, nanv, nanv}, nrm = vert;
       if (hit && tnear < tfar) {
         R t = tnear, fp;
-        V3<R> p = {o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z};
+        // points along the ray: o + t dir as fused multiply-adds
+        auto at = [&](R tt) { return V3<R>{std::fma(tt, dir.x, o.x), std::fma(tt, dir.y, o.y), std::fma(tt, dir.z, o.z)}; };
+        V3<R> p = at(t);
         bool ok = V.sample(p, fp) && fp >= R(0);
         for (int s = 0; ok && s < rp.max_steps; ++s) {
           ++total;
           const R st = (fp >= R(1)) ? step_far : step_near;
           const R tn = t + st;
           if (tn > tfar) break;
-          p = {o.x + tn * dir.x, o.y + tn * dir.y, o.z + tn * dir.z};
+          p = at(tn);
           R fc;
           if (!V.sample(p, fc)) break;
           if (fc < R(0)) {
             const R ts = t + st * (fp / (fp - fc));
-            const V3<R> ph = {o.x + ts * dir.x, o.y + ts * dir.y, o.z + ts * dir.z};
+            const V3<R> ph = at(ts);
             R gxp, gxm, gyp, gym, gzp, gzm;
             const bool g = V.sample({ph.x + V.h, ph.y, ph.z}, gxp) && V.sample({ph.x - V.h, ph.y, ph.z}, gxm) &&
                            V.sample({ph.x, ph.y + V.h, ph.z}, gyp) && V.sample({ph.x, ph.y - V.h, ph.z}, gym) &&
