@@ -173,15 +173,18 @@ __device__ __forceinline__ void halve(float* s, float* c, bool upper) {
 #pragma unroll
   for (int k = 0; k < H / 2; ++k) {
     const float ks = upper ? s[k + H / 2] : s[k];
-    const float kc = upper ? c[k + H / 2] : c[k];
     const float gs = upper ? s[k] : s[k + H / 2];
-    const float gc = upper ? c[k] : c[k + H / 2];
     const float rs = dpp_f<CTRL>(gs);
-    const float rc = COMP ? dpp_f<CTRL>(gc) : 0.0f;
-    float ns = ks, nc = kc;
-    combine<COMP>(ns, nc, rs, rc);
-    s[k] = ns;
-    c[k] = nc;
+    if constexpr (COMP) {  // (without compensation c stays zero and is not touched)
+      const float kc = upper ? c[k + H / 2] : c[k];
+      const float gc = upper ? c[k] : c[k + H / 2];
+      float ns = ks, nc = kc;
+      combine<COMP>(ns, nc, rs, dpp_f<CTRL>(gc));
+      s[k] = ns;
+      c[k] = nc;
+    } else {
+      s[k] = ks + rs;
+    }
   }
 }
 
@@ -281,8 +284,8 @@ __device__ __forceinline__ void block_partial(const IcpArgs& a, const kf_posef& 
   halve<0xB1, 4, COMP>(s, c, (lane & 1) != 0);
   {
     const bool up = (lane & 16) != 0;
-    const float gs = up ? s[0] : s[1], gc = up ? c[0] : c[1];
-    float ks = up ? s[1] : s[0], kc = up ? c[1] : c[0];
+    const float gs = up ? s[0] : s[1], gc = COMP ? (up ? c[0] : c[1]) : 0.0f;
+    float ks = up ? s[1] : s[0], kc = COMP ? (up ? c[1] : c[0]) : 0.0f;
     const float rs = __shfl_xor(gs, 16, kWave);
     const float rc = COMP ? __shfl_xor(gc, 16, kWave) : 0.0f;
     combine<COMP>(ks, kc, rs, rc);
