@@ -69,6 +69,29 @@ def test_overlapped_frames_match_in_order_frames(gpu, kf, synth, small):
     assert np.array_equal(out[0][1], out[1][1])
 
 
+def test_eager_frames_match_graph_replay(gpu, kf, synth, small):
+    """With a prior and no graphs, every call records its two streams eagerly (the
+    same fork, join and kernels as the replayed graph).  Same poses, same volume."""
+    print("prior pipeline: eager vs graph replay", flush=True)
+    prior = kf.Prior(max_batch=1)
+    prior.init(6)
+    F = 12
+    dd = kf.upload(small.depth[:F])
+    out = []
+    for use_graph in (1, 0):
+        d = synth.pipeline_desc("small")
+        d.use_graph = use_graph
+        p = kf.Pipeline(d)
+        p.attach_prior(prior)
+        p.reset(small.poses[0])
+        for i in range(F):
+            p.process(dd.ptr + i * small.depth[0].nbytes)
+        out.append((p.trajectory(), p.volume().download_raw(), p.stats().fell_back))
+        p.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_warm_started_tracking_without_fallback(gpu, kf, synth, small):
     print("prior pipeline: never fall back", flush=True)
     prior = kf.Prior(max_batch=1)
