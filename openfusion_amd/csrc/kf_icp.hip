@@ -426,26 +426,18 @@ __device__ __forceinline__ void wave_sync() {
 // `st`, thread 0 writes the level statistics and, when singular, status 1.
 // Called by the whole block; the first wavefront alone solves (its lanes stay in
 // step, so it needs no block barriers), the others wait at the final barrier.
-// Cholesky is right-looking and lane-parallel: at step j every lane takes the
-// pivot sqrt(W[j][j]) and its reciprocal itself, lanes j+1..5 write the column
-// below it and one lane per trailing entry subtracts the product of its two column
-// entries; every entry receives the same subtractions in the same order as the
-// serial left-looking loop (cholesky_solve6), so the factor has the same bits.
-// The substitutions are serial; the sines / cosines (3 lanes), the product (16
+// Lane 0 runs the serial Cholesky solve shared with the host (cholesky_solve6,
+// same bits) with the system in registers: a 6x6 solve is a short dependent chain,
+// cheaper than a lane-parallel factorisation that synchronises the wavefront
+// through LDS once per column.  The sines / cosines (3 lanes), the product (16
 // lanes) and the FP32 fields (24 lanes) are lane-parallel with the expressions of
 // the serial helpers.
 __device__ __forceinline__ bool solve_compose(const double* sum, const double* Tin, kf_icp_stats* st, int level,
                                               double* Tout, kf_posef* wc, kf_posef* cw) {
   __shared__ double tx[6], tcs[6], tD[16];
-  __shared__ double W[6][6], Lm[6][6], Li[6];
   __shared__ int tok;
   const int tid = threadIdx.x;
   if (tid < kWave) {
-    if (tid < 36) {
-      const int i = tid / 6, k = tid - 6 * (tid / 6);
-      const int r = i < k ? i : k, c = i < k ? k : i;
-      W[i][k] = sum[r * (11 - r) / 2 + c];
-    }
     const double inl = sum[28];
     const int ni = (int)inl;
     if (tid == 0 && st) {
@@ -455,48 +447,16 @@ __device__ __forceinline__ bool solve_compose(const double* sum, const double* T
       st->level_residual[level] = st->residual_rms;
       st->level_inliers[level] = ni;
     }
-    bool ok = ni >= kMinInliers;  // uniform over the wave from here on
-    wave_sync();
-    double thr = 0.0;
-    {
-      double dmax = 0.0;
-      for (int i = 0; i < 6; ++i) dmax = W[i][i] > dmax ? W[i][i] : dmax;
-      thr = kPivotRelEps * dmax;
-    }
-    for (int j = 0; j < 6 && ok; ++j) {
-      const double d = W[j][j];
-      if (!(d > thr)) {
-        ok = false;
-        break;
-      }
-      const double ljj = sqrt(d);
-      const double inv = 1.0 / ljj;
-      if (tid == j) {
-        Lm[j][j] = ljj;
-        Li[j] = inv;
-      } else if (tid > j && tid < 6) {
-        Lm[tid][j] = W[tid][j] * inv;
-      }
-      if (tid < 36) {
-        const int i = tid / 6, k = tid - 6 * (tid / 6);
-        if (k > j && i >= k) W[i][k] = W[i][k] - (W[i][j] * inv) * (W[k][j] * inv);
-      }
-      wave_sync();
-    }
     if (tid == 0) {
+      // the serial Cholesky solve shared with the host (same bits), in registers
+      double up[21], g[6], x[6];
+#pragma unroll
+      for (int q = 0; q < 21; ++q) up[q] = sum[q];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) g[q] = sum[21 + q];
+      const bool ok = ni >= kMinInliers && cholesky_solve6(up, g, kPivotRelEps, x);
       if (ok) {
-        const double* g = sum + 21;
-        double y[6], x[6];
-        for (int i = 0; i < 6; ++i) {
-          double s = g[i];
-          for (int m = 0; m < i; ++m) s = s - Lm[i][m] * y[m];
-          y[i] = s * Li[i];
-        }
-        for (int i = 5; i >= 0; --i) {
-          double s = y[i];
-          for (int m = i + 1; m < 6; ++m) s = s - Lm[m][i] * x[m];
-          x[i] = s * Li[i];
-        }
+#pragma unroll
         for (int q = 0; q < 6; ++q) tx[q] = x[q];
       } else if (st) {
         st->status = 1;
@@ -504,6 +464,7 @@ __device__ __forceinline__ bool solve_compose(const double* sum, const double* T
       tok = ok ? 1 : 0;
     }
     wave_sync();
+    const bool ok = tok != 0;  // uniform
     if (ok) {
       if (tid < 3) {
         const double ang = -tx[tid];  // rotation increment is the negated solution
