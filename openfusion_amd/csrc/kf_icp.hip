@@ -924,9 +924,10 @@ IcpArgs level_args(const kf_tracker* t, int l, const kf_frame_maps& live, const 
   return a;
 }
 
-// Tracking steps: 512-thread blocks, 4 / 2 / 1 pixels per thread at levels 0 / 1 / 2+.
-// Pixels per thread of a step's grid at level l (4 / 2 / 1).
-int step_ppt(int l) { return l == 0 ? 4 : (l == 1 ? 2 : 1); }
+// Tracking steps: 512-thread blocks, 4 pixels per thread at level 0 (150 blocks at
+// 640x480) and 1 below (150 / 38 blocks): on the coarse levels more blocks beat more
+// pixels per thread.
+int step_ppt(int l) { return l == 0 ? 4 : 1; }
 
 int step_blocks(const kf_tracker* t, int W, int H, int l) {
   int b = div_up(W * H, kStepThreads * step_ppt(l));
@@ -1022,11 +1023,9 @@ kf_status launch_icp_track(kf_tracker* t, const kf_frame_maps& live, const kf_fr
       const int K = step_ppt(l);
       if (p.compensated) {
         if (K == 4) hipLaunchKernelGGL((icp_step<true, 4>), dim3(blocks), dim3(kStepThreads), 0, s, a, S);
-        else if (K == 2) hipLaunchKernelGGL((icp_step<true, 2>), dim3(blocks), dim3(kStepThreads), 0, s, a, S);
         else hipLaunchKernelGGL((icp_step<true, 1>), dim3(blocks), dim3(kStepThreads), 0, s, a, S);
       } else {
         if (K == 4) hipLaunchKernelGGL((icp_step<false, 4>), dim3(blocks), dim3(kStepThreads), 0, s, a, S);
-        else if (K == 2) hipLaunchKernelGGL((icp_step<false, 2>), dim3(blocks), dim3(kStepThreads), 0, s, a, S);
         else hipLaunchKernelGGL((icp_step<false, 1>), dim3(blocks), dim3(kStepThreads), 0, s, a, S);
       }
       KF_TRY(check_launch("icp_step"));
