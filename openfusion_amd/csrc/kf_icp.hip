@@ -39,7 +39,7 @@ constexpr int kSums = kTerms + 3;  // slot sums: 28 terms, inliers, candidates, 
 constexpr int kThreads = 256;
 constexpr double kPivotRelEps = 1e-9;
 constexpr int kMinInliers = 6;
-constexpr int kFinUnroll = 8;       // partial-slot loads in flight per thread in the final sum
+constexpr int kFinUnroll = 5;       // slot loads in flight per thread of the slot sum (5 x 32 groups cover 150 slots in one round)
 constexpr int kStepThreads = 512;    // tracking step: 8 waves per block
 // One-block sums and solves use the step's block size, so that every slot sum (in
 // a step, the final one-block launch or the single-launch kernel) adds in the same
